@@ -1,0 +1,187 @@
+"""Benchmark of the random-forest training hot path on one MI355X.
+
+A step trains one whole tree of the mid-size configuration (500 synthetic
+320x240 uint16 depth images, 1M sampled pixels, 8 classes, 2000 candidate
+features x 20 thresholds per node, depth 16; tree ids cycle 0..2) in the
+level-batched form, with the training set already resident in HBM.
+
+value = sample*feature*threshold evaluations per second over the timed steps
+(evals = sum over levels of active samples * nF * nT).
+
+roofline: the dominant kernel is the node histogram.  Its algorithmic bytes
+per launch are
+    B = S*(4+4+1+b_px) + S*nF*2*b_px + nodes*nF*(FD+nT)*4 + nodes*nC*nT*nF*4
+(S active samples and `nodes` frontier nodes of the launch, b_px bytes per
+pixel); achieved = sum B / sum kernel time (HIP events on the trainer's
+stream), peak = 8 TB/s HBM.
+
+cpu_baseline: the CPU oracle (an op-for-op restatement of the reference
+histogram) timed on this host's cores on a bounded sample of the same
+workload: the root node of the same data set with a subset of the candidate
+features, sized to ~15 s.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def hist_bytes(S, nodes, nF, nT, nC, fd, b_px):
+    return S * (4 + 4 + 1 + b_px) + S * nF * 2 * b_px + nodes * nF * (fd + nT) * 4 + nodes * nC * nT * nF * 4
+
+
+def cpu_baseline(ts, params, cfg, target_s):
+    """Oracle root-node histogram on a subset of candidates, all host cores."""
+    import numpy as np
+
+    from oracle import oracle as orc
+    o = orc.Tset(cfg["kind"], ts)
+    S = o.total
+    sample_node = np.zeros(S, dtype=np.int32)
+    from padenti_amd import TreeTrainerParameters
+    nT = params.n_thresholds
+
+    def run(nf):
+        p = TreeTrainerParameters(nf, nT, params.feat_low_bounds, params.feat_up_bounds, params.thr_low_bound,
+                                  params.thr_up_bound, params.per_leaf_samples_thr)
+        t = time.perf_counter()
+        orc.hist_nodes(o, p, 0, [0], sample_node)
+        return time.perf_counter() - t
+
+    threads = orc.num_threads()
+    nf = max(threads, 8)
+    dt = run(nf)
+    nf2 = int(max(nf, min(params.n_features, nf * target_s / max(dt, 1e-3))))
+    nf2 = max(threads, (nf2 // threads) * threads)
+    dt2 = run(nf2)
+    return {
+        "value": S * nf2 * nT / dt2,
+        "unit": "evals/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"oracle histogram of the root node: {S} samples x {nf2} of {params.n_features} candidates "
+                  f"x {nT} thresholds ({dt2:.1f} s)",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c2")
+    ap.add_argument("--mode", default="batched", choices=["batched", "per_node"])
+    ap.add_argument("--images", type=int, default=None, help="scale the data set down (debug)")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--levels-json", default=None, help="write per-level records to this file")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (trees or candidate features "
+              "would shard across GPUs, see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import padenti_amd as pa
+    from padenti_amd import synth
+    if pa.device_count() < 1:
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    cfg = synth.CONFIGS[args.config]
+    t0 = time.perf_counter()
+    ts = synth.training_set(args.config, n_images=args.images)
+    params = synth.params_for(args.config)
+    gen_s = time.perf_counter() - t0
+    print(f"[bench] generated {len(ts.images)} images in {gen_s:.1f} s", file=sys.stderr, flush=True)
+    mode = pa.PD_MODE_LEVEL_BATCHED if args.mode == "batched" else pa.PD_MODE_PER_NODE
+    tr = pa.HIPTreeTrainer(cfg["kind"], cfg["n_classes"])
+    tr.set_training_set(ts)
+    depth = cfg["depth"]
+    fd = pa.FEATURES[cfg["kind"]][3]
+    b_px = 2 if cfg["kind"] != pa.PD_FEAT_DEPTH_F32 else 4
+    if cfg["kind"] == pa.PD_FEAT_RGBD_U16X4:
+        b_px = 8
+
+    def step(i):
+        tree = pa.Tree(i % cfg["trees"], depth, cfg["kind"], cfg["n_classes"])
+        t = time.perf_counter()
+        tr.train(tree, ts, params, 1, depth, mode=mode)
+        pa.lib().pd_dev_sync(0)
+        return time.perf_counter() - t, tr.last_level_stats, tree
+
+    for i in range(args.warmup):
+        dt, _, _ = step(i)
+        print(f"[bench] warmup tree {i}: {dt:.3f} s", file=sys.stderr, flush=True)
+    pa.lib().pd_dev_sync(0)
+    t_all = time.perf_counter()
+    evals = 0.0
+    hist_s = 0.0
+    hist_b = 0.0
+    launches = 0
+    levels = []
+    for i in range(args.steps):
+        dt, st, tree = step(i)
+        e = sum(s["evals"] for s in st)
+        evals += e
+        for s in st:
+            hist_s += s["hist_ms"] / 1e3
+            launches += s["hist_launches"]
+            hist_b += hist_bytes(s["active_samples"], s["frontier_nodes"], params.n_features, params.n_thresholds,
+                                 cfg["n_classes"], fd, b_px)
+        levels.append(st)
+        print(f"[bench] tree {i % cfg['trees']}: {dt:.3f} s, {e / dt / 1e12:.2f} Teval/s, "
+              f"splits {int((tree.left_children >= 0).sum())}", file=sys.stderr, flush=True)
+    pa.lib().pd_dev_sync(0)
+    total = time.perf_counter() - t_all
+    value = evals / total
+    achieved = hist_b / hist_s / 1e9 if hist_s > 0 else 0.0
+    if args.levels_json:
+        with open(args.levels_json, "w") as f:
+            json.dump(levels, f, indent=1)
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(ts, params, cfg, args.cpu_seconds)
+        print(f"[bench] cpu baseline {cpu['value'] / 1e9:.3f} Geval/s on {cpu['cores']} threads",
+              file=sys.stderr, flush=True)
+    out = {
+        "metric": "sample*feature*threshold evals/s (plain level-batched tree training)",
+        "value": value,
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": total / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.config}: train one depth-{depth} tree per step, {len(ts.images)} images "
+                        f"{cfg['w']}x{cfg['h']}, {sum(im.samples.size for im in ts.images)} samples, "
+                        f"{params.n_features} features x {params.n_thresholds} thresholds, "
+                        f"{cfg['n_classes']} classes, {args.mode}",
+            "tree_train_s": total / args.steps,
+            "hist_kernel_launches": launches,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    tr.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,183 @@
+/*
+ * padenti_hip.h -- C ABI of the MI355X (gfx950) random-forest hot path.
+ *
+ * Plain pointers and sizes only.  Every entry point returns PD_OK (0) or a
+ * negative PD_ERR_* code; pd_last_error() gives a thread-local message.
+ * The C++ drop-in API (include/padenti/hip_tree_trainer.hpp,
+ * include/padenti/hip_classifier.hpp) is a thin layer over this header.
+ *
+ * Reference interfaces replaced (paths relative to the reference root):
+ *   CLTreeTrainer ctor       src/padenti/cl_tree_trainer_impl.hpp:73-198
+ *   CLTreeTrainer::train     src/padenti/cl_tree_trainer_impl.hpp:203-255
+ *                            (+ _initTrain, _traverseTrainingSet,
+ *                               _learnBestFeatThr)
+ *   computePerImageHistogram src/padenti/kernels/hist_update.cl:36-187
+ *                            + host accumulation
+ *                            src/padenti/cl_tree_trainer_impl_traverse_ts.hpp:462-704
+ *   learnBestFeature         src/padenti/kernels/learn_best_feature.cl:22-111
+ *                            + first max cl_tree_trainer_impl_learn_best_featthr.hpp:101-112
+ *   CLClassifier ctor / <<   src/padenti/cl_classifier_impl.hpp:41-195
+ *   CLClassifier::predict    src/padenti/cl_classifier_impl.hpp:198-557
+ *                            (predict.cl:36-117)
+ */
+#ifndef PADENTI_HIP_H
+#define PADENTI_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PD_OK 0
+#define PD_ERR_ARG -1      /* invalid argument / shape */
+#define PD_ERR_HIP -2      /* HIP runtime error */
+#define PD_ERR_STATE -3    /* call out of order (e.g. train before set_data) */
+#define PD_ERR_NOMEM -4    /* device allocation failed */
+#define PD_ERR_PARITY -5   /* internal consistency check failed */
+
+/* Feature hooks compiled into the library (HIP functors standing in for
+ * the reference's feature.cl plugin, test/feature.cl:29-34). */
+#define PD_FEAT_DEPTH_F32 0  /* ImgType float x1, FeatType float, FeatDim 4 */
+#define PD_FEAT_DEPTH_U16 1  /* ImgType uint16 x1, FeatType float, FeatDim 4 */
+#define PD_FEAT_RGBD_U16X4 2 /* ImgType uint16 x4, FeatType float, FeatDim 6 */
+#define PD_FEAT_DEMO_S16 3   /* ImgType uint16 x1, FeatType short, FeatDim 2 */
+
+#define PD_MODE_PER_NODE 0      /* one histogram + one split launch per node */
+#define PD_MODE_LEVEL_BATCHED 1 /* one launch per tree level (per slice) */
+
+/* Mirrors TreeTrainerParameters<FeatType, FeatDim> (src/padenti/tree_trainer.hpp:35-51).
+ * Bounds are carried as double and converted to FeatType (exact for float
+ * and short).  computeFRange / randomThrSampling are unused in the
+ * reference and omitted. */
+typedef struct {
+  uint32_t n_features;
+  uint32_t n_thresholds;
+  double feat_low[8];
+  double feat_up[8];
+  double thr_low;
+  double thr_up;
+  uint32_t per_leaf_samples_thr;
+  uint32_t reserved;
+} pd_train_params;
+
+/* Per-level record filled by pd_trainer_train (the reference logs the same
+ * quantities with BOOST_LOG_TRIVIAL, cl_tree_trainer_impl.hpp:249-250 and
+ * cl_tree_trainer_impl_traverse_ts.hpp:437-454). */
+typedef struct {
+  uint32_t level;
+  uint32_t frontier_nodes;
+  uint32_t slices;
+  uint32_t hist_launches;
+  uint64_t active_samples;  /* samples in trainable frontier nodes */
+  double evals;             /* active_samples * nF * nT */
+  double hist_ms;           /* device time of the histogram kernel(s) */
+  double split_ms;          /* device time of split + finalize */
+  double route_ms;          /* device time of routing */
+  double wall_ms;           /* host wall clock of the level */
+  uint32_t split_nodes;
+  uint32_t reserved;
+} pd_level_stats;
+
+typedef struct pd_trainer pd_trainer;
+typedef struct pd_forest pd_forest;
+
+const char *pd_last_error(void);
+int pd_version(void);
+/* Number of visible HIP devices (0 without a GPU; never fails). */
+int pd_device_count(void);
+
+/* ---- training (replaces CLTreeTrainer, cl_tree_trainer.hpp:115) ----- */
+int pd_trainer_create(pd_trainer **out, int device, int feature_kind, uint32_t n_classes);
+int pd_trainer_destroy(pd_trainer *tr);
+
+/* Upload the training set (TrainingSetImage list, training_set_image.hpp:64-66).
+ * pixels: all images packed, img_offsets[i] = element offset of image i;
+ * labels: packed label images (uint8, 0 = unlabelled, 1..nC), label_offsets
+ * in bytes; samples: packed pixel indices y*w+x, sample_offsets in elements.
+ * Every sample must be labelled (the reference indexes label-1). */
+int pd_trainer_set_data(pd_trainer *tr, uint32_t n_images, const void *pixels, const uint64_t *img_offsets,
+                        const uint32_t *widths, const uint32_t *heights, const uint8_t *labels,
+                        const uint64_t *label_offsets, const uint32_t *samples, const uint64_t *sample_offsets,
+                        const uint32_t *n_samples);
+
+/* Device memory budget for histograms (bytes); frontier levels whose
+ * histograms exceed it are processed in slices (the reference slices at
+ * GLOBAL_HISTOGRAM_MAX_SIZE of host RAM, cl_tree_trainer_impl.hpp:50). */
+int pd_trainer_set_hist_budget(pd_trainer *tr, uint64_t bytes);
+/* Samples per histogram work item (0 = automatic). */
+int pd_trainer_set_chunk(pd_trainer *tr, uint32_t samples_per_chunk);
+
+/* Train one tree: TreeTrainer::train(tree, ts, params, 1, end_depth)
+ * (tree_trainer.hpp:82-85).  Tree arrays are the reference SoA layout of a
+ * tree of depth `tree_depth` (N = 2^tree_depth - 1 nodes): left_children[N],
+ * features[N*FeatDim], thresholds[N] (FeatType), posteriors[N*nC]; they are
+ * overwritten.  root_prior[nC] is TrainingSet::getPriors().  Optional
+ * per-node outputs (size N, may be NULL): chosen k = t*nF + f, FP32 gain,
+ * FP64 gain of the chosen split.  level_stats (may be NULL) receives
+ * end_depth-1 records. */
+int pd_trainer_train(pd_trainer *tr, const pd_train_params *params, uint32_t tree_id, uint32_t tree_depth,
+                     uint32_t end_depth, int mode, const float *root_prior, int32_t *left_children,
+                     void *features, void *thresholds, float *posteriors, uint32_t *node_k, float *node_gain,
+                     double *node_gain64, pd_level_stats *level_stats);
+
+/* (a) alone: histograms H[n][c][t][f] (uint32) of the listed nodes, with
+ * sample_node[s] giving the node of every packed sample (-1 = none). */
+int pd_trainer_hist_nodes(pd_trainer *tr, const pd_train_params *params, uint32_t tree_id, uint32_t n_nodes,
+                          const int32_t *node_ids, const int32_t *sample_node, uint32_t *H_out);
+
+/* (b) alone: for each node, the first maximum over k = t*nF + f of the FP32
+ * gain, from host H[n][c][t][f] and per-class totals tot[n][c]. */
+int pd_split_search(int device, uint32_t n_nodes, uint32_t n_classes, uint32_t n_thresholds,
+                    uint32_t n_features, const uint32_t *H, const uint32_t *tot, uint32_t *best_k,
+                    float *best_gain, float *all_gains);
+
+/* Candidate tables on the device: params[n][f][FeatDim], thresholds[n][f][nT]. */
+int pd_candidate_tables(int device, int feature_kind, const pd_train_params *params, uint32_t tree_id,
+                        uint32_t n_nodes, const int32_t *node_ids, void *params_out, void *thr_out);
+
+/* Unit entry points (device evaluation of single steps). */
+int pd_md5rand(int device, uint32_t n, const uint32_t *seeds, uint32_t *out);
+int pd_log2_det(int device, uint32_t n, const float *x, float *y);
+int pd_feature_eval(int device, int feature_kind, const void *image, uint32_t w, uint32_t h, uint32_t n,
+                    const int32_t *xy, const void *params, void *out);
+
+/* ---- prediction (replaces CLClassifier, cl_classifier.hpp:81-94) ----- */
+int pd_forest_create(pd_forest **out, int device, int feature_kind, uint32_t n_classes);
+int pd_forest_destroy(pd_forest *fo);
+/* CLClassifier::operator<<(const Tree&): copies the tree to the device. */
+int pd_forest_add_tree(pd_forest *fo, uint32_t depth, const int32_t *left_children, const void *features,
+                       const void *thresholds, const float *posteriors);
+int pd_forest_n_trees(pd_forest *fo);
+/* predict(id, image, leafIdx[, mask]): out[h*w] leaf ids, -1 where mask==0. */
+int pd_forest_predict_leaves(pd_forest *fo, uint32_t tree_index, const void *image, uint32_t w, uint32_t h,
+                             const uint8_t *mask, int32_t *out);
+/* predict(image, posterior[, mask]): out[nC][h][w] planar FP32. */
+int pd_forest_predict_posterior(pd_forest *fo, const void *image, uint32_t w, uint32_t h, const uint8_t *mask,
+                                float *out);
+
+/* Batched multi-frame prediction on device-resident buffers (one launch,
+ * optionally replayed from a captured hipGraph).  d_pixels: frames packed,
+ * img_offsets[i] element offsets; d_masks may be NULL (else byte offsets
+ * mask_offsets).  Outputs (either may be NULL): d_post, frame i at element
+ * post_offsets[i] as [nC][h][w]; d_leaves, frame i at leaf_offsets[i] as
+ * [tree][h][w].  use_graph != 0 captures the launch once per shape set and
+ * replays it.  *kernel_ms (may be NULL) receives the device time. */
+int pd_forest_predict_frames_dev(pd_forest *fo, uint32_t n_frames, const void *d_pixels,
+                                 const uint64_t *img_offsets, const uint32_t *widths, const uint32_t *heights,
+                                 const uint8_t *d_masks, const uint64_t *mask_offsets, float *d_post,
+                                 const uint64_t *post_offsets, int32_t *d_leaves, const uint64_t *leaf_offsets,
+                                 int use_graph, float *kernel_ms);
+
+/* ---- device memory helpers (for callers without another allocator) ---- */
+int pd_dev_alloc(int device, uint64_t bytes, void **out);
+int pd_dev_free(void *p);
+int pd_memcpy_h2d(void *dst, const void *src, uint64_t bytes);
+int pd_memcpy_d2h(void *dst, const void *src, uint64_t bytes);
+int pd_dev_sync(int device);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PADENTI_HIP_H */

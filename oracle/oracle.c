@@ -1,0 +1,685 @@
+/*
+ * CPU oracle for the per-pixel random-forest hot path of Padenti.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is a plain-C restatement of the
+ * reference algorithm and exists to check the HIP implementation.  Only the
+ * test-suite, the smoke check and the CPU-baseline leg of the benchmark may
+ * load it; the product library never links or calls it.
+ *
+ * Every function cites the reference file:line it restates (paths relative
+ * to the reference repository root).  Build flags: -O2 -ffp-contract=off
+ * (no fused multiply-add anywhere: the float formulas below must round after
+ * every operation, as the reference's op order is written).
+ *
+ * Parity status of each part (how it is pinned, see tests/test_oracle_*.py):
+ *   or_md5rand          pinned: RFC 1321 MD5 (hashlib) and the reference's own
+ *                       md5Rand compiled from src/padenti/prng.hpp (oracle/_ref)
+ *   or_sample_*         pinned: closed form from MD5 + formula, numpy float32
+ *   or_log2_det         pinned: |log2_det - log2| <= 1 ulp (numpy), exact at 2^k
+ *   or_gain_f32         pinned: FP64 closed form (1e-5 abs), constructed ties
+ *   features            pinned: numpy brute force incl. out-of-image and .5 rounding
+ *   or_level_hist       pinned: brute-force count definition (numpy, tiny inputs)
+ *   or_train_tree       pinned: invariants (left+right=parent, stop rules, root
+ *                       always trained, child indices) + the pinned parts above
+ *   or_predict_*        pinned: numpy brute-force traversal, FP64 averages
+ */
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------ */
+/* A1: md5Rand = MD5 of the 16 little-endian bytes of the seed.        */
+/* Restates src/padenti/kernels/hist_update.cl:278-369 and             */
+/* src/padenti/prng.hpp:107-195 (one MD5 block: seed words, 0x80 pad,  */
+/* bit length 128 in word 14).  Written from RFC 1321.                 */
+/* ------------------------------------------------------------------ */
+static const uint32_t MD5_K[64] = {
+  0xd76aa478u, 0xe8c7b756u, 0x242070dbu, 0xc1bdceeeu, 0xf57c0fafu, 0x4787c62au,
+  0xa8304613u, 0xfd469501u, 0x698098d8u, 0x8b44f7afu, 0xffff5bb1u, 0x895cd7beu,
+  0x6b901122u, 0xfd987193u, 0xa679438eu, 0x49b40821u, 0xf61e2562u, 0xc040b340u,
+  0x265e5a51u, 0xe9b6c7aau, 0xd62f105du, 0x02441453u, 0xd8a1e681u, 0xe7d3fbc8u,
+  0x21e1cde6u, 0xc33707d6u, 0xf4d50d87u, 0x455a14edu, 0xa9e3e905u, 0xfcefa3f8u,
+  0x676f02d9u, 0x8d2a4c8au, 0xfffa3942u, 0x8771f681u, 0x6d9d6122u, 0xfde5380cu,
+  0xa4beea44u, 0x4bdecfa9u, 0xf6bb4b60u, 0xbebfbc70u, 0x289b7ec6u, 0xeaa127fau,
+  0xd4ef3085u, 0x04881d05u, 0xd9d4d039u, 0xe6db99e5u, 0x1fa27cf8u, 0xc4ac5665u,
+  0xf4292244u, 0x432aff97u, 0xab9423a7u, 0xfc93a039u, 0x655b59c3u, 0x8f0ccc92u,
+  0xffeff47du, 0x85845dd1u, 0x6fa87e4fu, 0xfe2ce6e0u, 0xa3014314u, 0x4e0811a1u,
+  0xf7537e82u, 0xbd3af235u, 0x2ad7d2bbu, 0xeb86d391u};
+static const int MD5_R[64] = {7, 12, 17, 22, 7, 12, 17, 22, 7, 12, 17, 22, 7, 12, 17, 22,
+                              5, 9,  14, 20, 5, 9,  14, 20, 5, 9,  14, 20, 5, 9,  14, 20,
+                              4, 11, 16, 23, 4, 11, 16, 23, 4, 11, 16, 23, 4, 11, 16, 23,
+                              6, 10, 15, 21, 6, 10, 15, 21, 6, 10, 15, 21, 6, 10, 15, 21};
+
+void or_md5rand(const uint32_t seed[4], uint32_t out[4])
+{
+  uint32_t m[16];
+  memset(m, 0, sizeof(m));
+  m[0] = seed[0]; m[1] = seed[1]; m[2] = seed[2]; m[3] = seed[3];
+  m[4] = 0x80u;   /* padding byte after the 16 message bytes */
+  m[14] = 128u;   /* message length in bits */
+  uint32_t a = 0x67452301u, b = 0xefcdab89u, c = 0x98badcfeu, d = 0x10325476u;
+  for (int i = 0; i < 64; i++) {
+    uint32_t f; int g;
+    if (i < 16)      { f = (b & c) | (~b & d); g = i; }
+    else if (i < 32) { f = (d & b) | (~d & c); g = (5 * i + 1) & 15; }
+    else if (i < 48) { f = b ^ c ^ d;          g = (3 * i + 5) & 15; }
+    else             { f = c ^ (b | ~d);       g = (7 * i) & 15; }
+    uint32_t t = a + f + MD5_K[i] + m[g];
+    a = d; d = c; c = b;
+    b = b + ((t << MD5_R[i]) | (t >> (32 - MD5_R[i])));
+  }
+  out[0] = a + 0x67452301u; out[1] = b + 0xefcdab89u;
+  out[2] = c + 0x98badcfeu; out[3] = d + 0x10325476u;
+}
+
+/* ------------------------------------------------------------------ */
+/* Feature kinds.  Each is one instantiation of the reference's        */
+/* <ImgType, nChannels, FeatType, FeatDim> template set plus a         */
+/* computeFeature hook (contract: test/feature.cl:29-34).              */
+/* ------------------------------------------------------------------ */
+enum {
+  OR_FEAT_DEPTH_F32 = 0,   /* float metres, FeatType float, FD 4 */
+  OR_FEAT_DEPTH_U16 = 1,   /* uint16 mm, FeatType float, FD 4 */
+  OR_FEAT_RGBD_U16X4 = 2,  /* uint16 x4 (R,G,B,D mm), FeatType float, FD 6 */
+  OR_FEAT_DEMO_S16 = 3     /* uint16 mm, FeatType short, FD 2 (test/feature.cl) */
+};
+
+static int feat_dim_of(int kind)
+{
+  switch (kind) {
+    case OR_FEAT_DEPTH_F32: return 4;
+    case OR_FEAT_DEPTH_U16: return 4;
+    case OR_FEAT_RGBD_U16X4: return 6;
+    case OR_FEAT_DEMO_S16: return 2;
+  }
+  return -1;
+}
+static int feat_is_short(int kind) { return kind == OR_FEAT_DEMO_S16; }
+
+/* A2/A3: lo + (T)(((float)w)/0xFFFFFFFF * (hi - lo))
+ * hist_update.cl:94-111 (params), :150-183 (thresholds);
+ * host mirror cl_tree_trainer_impl_learn_best_featthr.hpp:192-222.
+ * (float)0xFFFFFFFF rounds to 2^32, so the quotient is w/2^32 exactly. */
+float or_sample_f32(uint32_t w, float lo, float hi)
+{
+  float u = (float)w / (float)0xFFFFFFFFu;
+  float span = hi - lo;
+  float q = u * span;
+  return lo + q;
+}
+int16_t or_sample_s16(uint32_t w, int16_t lo, int16_t hi)
+{
+  float u = (float)w / (float)0xFFFFFFFFu;
+  int span = (int)hi - (int)lo;
+  float q = u * (float)span;
+  int16_t qi = (int16_t)q;            /* float -> short truncates */
+  return (int16_t)((int)lo + (int)qi);
+}
+
+/* Parameters of candidate f of node n of tree id: seed (id, n, f, 0),
+ * chain seed <- md5(seed) once per 4 entries (hist_update.cl:62-114). */
+void or_feature_params(int kind, uint32_t tree, uint32_t node, uint32_t f,
+                       const double *lo, const double *hi, void *out)
+{
+  int fd = feat_dim_of(kind);
+  uint32_t seed[4] = {tree, node, f, 0u}, st[4];
+  for (int j = 0; j < fd; j++) {
+    if ((j & 3) == 0) { or_md5rand(seed, st); memcpy(seed, st, sizeof(st)); }
+    if (feat_is_short(kind))
+      ((int16_t *)out)[j] = or_sample_s16(seed[j & 3], (int16_t)lo[j], (int16_t)hi[j]);
+    else
+      ((float *)out)[j] = or_sample_f32(seed[j & 3], (float)lo[j], (float)hi[j]);
+  }
+}
+
+/* Thresholds of candidate f of node n: seed (id, n, f, 1), one md5 per 4
+ * thresholds (hist_update.cl:127-184; host :211-222). */
+void or_thresholds(int kind, uint32_t tree, uint32_t node, uint32_t f, uint32_t nT,
+                   double lo, double hi, void *out)
+{
+  uint32_t seed[4] = {tree, node, f, 1u}, st[4];
+  for (uint32_t t = 0; t < nT; t++) {
+    if ((t & 3) == 0) { or_md5rand(seed, st); memcpy(seed, st, sizeof(st)); }
+    if (feat_is_short(kind))
+      ((int16_t *)out)[t] = or_sample_s16(seed[t & 3], (int16_t)lo, (int16_t)hi);
+    else
+      ((float *)out)[t] = or_sample_f32(seed[t & 3], (float)lo, (float)hi);
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* A5: feature hooks.                                                  */
+/* ------------------------------------------------------------------ */
+typedef struct {
+  const void *data;   /* interleaved pixels, row-major */
+  int w, h, nch;
+} or_image;
+
+/* Offset = round(u / d): roundf is half-away-from-zero as OpenCL round()
+ * (test/feature.cl:50).  A quotient whose magnitude is not below 2^20 is
+ * treated as leaving the image (keeps the float->int conversion defined). */
+static int offset_ok(float q) { return fabsf(q) < 1048576.0f; }
+
+static float px_f32(const or_image *im, int x, int y, int c)
+{ return ((const float *)im->data)[((size_t)y * im->w + x) * im->nch + c]; }
+static uint16_t px_u16(const or_image *im, int x, int y, int c)
+{ return ((const uint16_t *)im->data)[((size_t)y * im->w + x) * im->nch + c]; }
+
+/* Config C1 hook: f = I(x + round(u/d(x))) - I(x + round(v/d(x))), float
+ * metres, out-of-image reads return 10 m; d(x) <= 0 gives f = 0. */
+static float feat_depth_f32(const or_image *im, int x, int y, const float *p)
+{
+  const float BG = 10.0f;
+  float d = px_f32(im, x, y, 0);
+  if (!(d > 0.0f)) return 0.0f;
+  float r[2];
+  for (int k = 0; k < 2; k++) {
+    float qx = roundf(p[2 * k] / d), qy = roundf(p[2 * k + 1] / d);
+    if (!offset_ok(qx) || !offset_ok(qy)) { r[k] = BG; continue; }
+    int ox = x + (int)qx, oy = y + (int)qy;
+    r[k] = (ox < 0 || ox >= im->w || oy < 0 || oy >= im->h) ? BG : px_f32(im, ox, oy, 0);
+  }
+  return r[0] - r[1];
+}
+
+/* Configs C2/C3 hook: same with uint16 millimetres; u, v in pixel*mm;
+ * out-of-image reads return 10000 mm; response is float. */
+static float feat_depth_u16(const or_image *im, int x, int y, const float *p)
+{
+  const float BG = 10000.0f;
+  float d = (float)px_u16(im, x, y, 0);
+  if (!(d > 0.0f)) return 0.0f;
+  float r[2];
+  for (int k = 0; k < 2; k++) {
+    float qx = roundf(p[2 * k] / d), qy = roundf(p[2 * k + 1] / d);
+    if (!offset_ok(qx) || !offset_ok(qy)) { r[k] = BG; continue; }
+    int ox = x + (int)qx, oy = y + (int)qy;
+    r[k] = (ox < 0 || ox >= im->w || oy < 0 || oy >= im->h) ? BG : (float)px_u16(im, ox, oy, 0);
+  }
+  return r[0] - r[1];
+}
+
+/* Config C4 hook: depth-normalised offset channel difference over a
+ * channel pair.  Pixel = (R, G, B, D) uint16; p = (ux, uy, vx, vy, ca, cb);
+ * channel index = (int)c clamped to [0, 3]; d = channel 3. */
+static int chan_of(float c)
+{
+  if (!(c > 0.0f)) return 0;
+  if (c >= 3.0f) return 3;
+  return (int)c;
+}
+static float feat_rgbd(const or_image *im, int x, int y, const float *p)
+{
+  const float BG = 10000.0f;
+  float d = (float)px_u16(im, x, y, 3);
+  if (!(d > 0.0f)) return 0.0f;
+  int ch[2] = {chan_of(p[4]), chan_of(p[5])};
+  float r[2];
+  for (int k = 0; k < 2; k++) {
+    float qx = roundf(p[2 * k] / d), qy = roundf(p[2 * k + 1] / d);
+    if (!offset_ok(qx) || !offset_ok(qy)) { r[k] = BG; continue; }
+    int ox = x + (int)qx, oy = y + (int)qy;
+    r[k] = (ox < 0 || ox >= im->w || oy < 0 || oy >= im->h) ? BG : (float)px_u16(im, ox, oy, ch[k]);
+  }
+  return r[0] - r[1];
+}
+
+/* The reference demo hook, test/feature.cl:29-64 with feat_t = short:
+ * depth = (short)I(x); offset = x + (int)round((float)u*500.0f/depth);
+ * out-of-image -> 10000; response = depth - (d2 ? d2 : 10000) in short.
+ * depth == 0 (a division by zero in the reference) gives 0 here. */
+static int16_t feat_demo_s16(const or_image *im, int x, int y, const int16_t *p)
+{
+  int16_t depth = (int16_t)px_u16(im, x, y, 0);
+  if (depth == 0) return 0;
+  int16_t response = depth;
+  float qx = roundf((float)p[0] * 500.0f / (float)depth);
+  float qy = roundf((float)p[1] * 500.0f / (float)depth);
+  int16_t d2;
+  if (!offset_ok(qx) || !offset_ok(qy)) {
+    d2 = (int16_t)10000;
+  } else {
+    int ox = x + (int)qx, oy = y + (int)qy;
+    int oob = ox < 0 || ox >= im->w || oy < 0 || oy >= im->h;
+    d2 = oob ? (int16_t)10000 : (int16_t)px_u16(im, ox, oy, 0);
+  }
+  response = (int16_t)(response - (d2 ? d2 : (int16_t)10000));
+  return response;
+}
+
+/* Evaluate hook `kind`; returns the response widened to double (exact for
+ * float and short).  `p` holds FeatType params. */
+double or_feature(int kind, const or_image *im, int x, int y, const void *p)
+{
+  switch (kind) {
+    case OR_FEAT_DEPTH_F32: return feat_depth_f32(im, x, y, (const float *)p);
+    case OR_FEAT_DEPTH_U16: return feat_depth_u16(im, x, y, (const float *)p);
+    case OR_FEAT_RGBD_U16X4: return feat_rgbd(im, x, y, (const float *)p);
+    case OR_FEAT_DEMO_S16: return feat_demo_s16(im, x, y, (const int16_t *)p);
+  }
+  return 0.0;
+}
+
+/* feat <= thr in FeatType (hist_update.cl:154; predict.cl:75) */
+static int goes_left(int kind, double v, const void *thr, uint32_t t)
+{
+  if (feat_is_short(kind)) return (int16_t)v <= ((const int16_t *)thr)[t];
+  return (float)v <= ((const float *)thr)[t];
+}
+
+/* C-ABI wrapper for tests: evaluate a hook at (x, y) on one image. */
+double or_feature_at(int kind, const void *data, int w, int h, int nch, int x, int y,
+                     const void *params)
+{
+  or_image im = {data, w, h, nch};
+  return or_feature(kind, &im, x, y, params);
+}
+
+/* ------------------------------------------------------------------ */
+/* log2 used by the information gain.  The reference calls OpenCL      */
+/* log2 (<= 3 ulp, device dependent; learn_best_feature.cl:85-91).     */
+/* The project fixes one deterministic function instead:               */
+/*   x = m * 2^e with m in [sqrt(1/2), sqrt(2)),                       */
+/*   s = (m-1)/(m+1), ln m = 2 s (1 + s^2/3 + ... + s^22/23)  (FP64),   */
+/*   log2 x = (float)(e + ln m * log2(e)).                             */
+/* Only IEEE add/mul/div in a fixed order, so host and device agree.   */
+/* ------------------------------------------------------------------ */
+float or_log2_det(float x)
+{
+  uint32_t b;
+  memcpy(&b, &x, 4);
+  int e = (int)((b >> 23) & 0xffu) - 127;
+  uint32_t mb = (b & 0x007fffffu) | 0x3f800000u;
+  float mf;
+  memcpy(&mf, &mb, 4);
+  double m = (double)mf;
+  if (m > 1.4142135623730951) { m = m * 0.5; e += 1; }
+  double s = (m - 1.0) / (m + 1.0);
+  double s2 = s * s;
+  double poly = 1.0 / 23.0;
+  poly = poly * s2 + 1.0 / 21.0;
+  poly = poly * s2 + 1.0 / 19.0;
+  poly = poly * s2 + 1.0 / 17.0;
+  poly = poly * s2 + 1.0 / 15.0;
+  poly = poly * s2 + 1.0 / 13.0;
+  poly = poly * s2 + 1.0 / 11.0;
+  poly = poly * s2 + 1.0 / 9.0;
+  poly = poly * s2 + 1.0 / 7.0;
+  poly = poly * s2 + 1.0 / 5.0;
+  poly = poly * s2 + 1.0 / 3.0;
+  poly = poly * s2 + 1.0;
+  double lnm = 2.0 * s * poly;
+  double r = (double)e + lnm * 1.4426950408889634;
+  return (float)r;
+}
+
+/* ------------------------------------------------------------------ */
+/* A9: FP32 information gain of one (threshold, feature) pair,         */
+/* op for op as learn_best_feature.cl:42-93 (EPS = 1.e-6 is a double). */
+/* left[c] = H[c][t][f], tot[c] = per-class totals of the node.        */
+/* ------------------------------------------------------------------ */
+float or_gain_f32(const uint32_t *left, const uint32_t *tot, uint32_t nC)
+{
+  float h = 0.0f, hL = 0.0f, hR = 0.0f, tmp;
+  int nL = 0, nR = 0;
+  for (uint32_t l = 0; l < nC; l++) {
+    nL += (int)left[l];
+    nR += (int)(tot[l] - left[l]);
+  }
+  for (uint32_t l = 0; l < nC; l++) {
+    tmp = ((float)tot[l]) / (float)(nL + nR);
+    h -= ((double)tmp <= 1.e-6) ? 0.0f : tmp * or_log2_det(tmp);
+    tmp = nL ? (float)left[l] / (float)nL : 0.0f;
+    hL -= ((double)tmp <= 1.e-6) ? 0.0f : tmp * or_log2_det(tmp);
+    tmp = nR ? (float)(tot[l] - left[l]) / (float)nR : 0.0f;
+    hR -= ((double)tmp <= 1.e-6) ? 0.0f : tmp * or_log2_det(tmp);
+  }
+  float wl = (float)nL / (float)(nL + nR);
+  float wr = (float)nR / (float)(nL + nR);
+  float a = wl * hL;
+  float b2 = wr * hR;
+  float s = a + b2;
+  return h - s;
+}
+
+/* The same gain in FP64 with libm log2 (the FP64 host oracle). */
+double or_gain_f64(const uint32_t *left, const uint32_t *tot, uint32_t nC)
+{
+  double h = 0, hL = 0, hR = 0, nL = 0, nR = 0;
+  for (uint32_t l = 0; l < nC; l++) { nL += left[l]; nR += (double)(tot[l] - left[l]); }
+  double N = nL + nR;
+  for (uint32_t l = 0; l < nC; l++) {
+    double p = tot[l] / N;
+    if (p > 1e-6) h -= p * log2(p);
+    p = nL > 0 ? left[l] / nL : 0.0;
+    if (p > 1e-6) hL -= p * log2(p);
+    p = nR > 0 ? (tot[l] - left[l]) / nR : 0.0;
+    if (p > 1e-6) hR -= p * log2(p);
+  }
+  return h - (nL / N * hL + nR / N * hR);
+}
+
+/* First maximum over k = t*nF + f (threshold-major).  The reference
+ * keeps a per-thread first max over 64 consecutive k with a strict '>'
+ * (learn_best_feature.cl:54-59, 96-101) and std::max_element across
+ * threads (cl_tree_trainer_impl_learn_best_featthr.hpp:111-112); both
+ * together are the first maximum in k order. */
+uint32_t or_best_split(const uint32_t *H, const uint32_t *tot, uint32_t nC, uint32_t nT,
+                       uint32_t nF, float *best_gain)
+{
+  uint32_t *left = (uint32_t *)malloc(sizeof(uint32_t) * nC);
+  uint32_t best = 0;
+  float bg = 0.0f;
+  for (uint32_t k = 0; k < nT * nF; k++) {
+    for (uint32_t c = 0; c < nC; c++) left[c] = H[(size_t)c * nT * nF + k];
+    float g = or_gain_f32(left, tot, nC);
+    if (k == 0 || g > bg) { bg = g; best = k; }
+  }
+  free(left);
+  if (best_gain) *best_gain = bg;
+  return best;
+}
+
+/* ------------------------------------------------------------------ */
+/* Training set description (all images packed).                      */
+/* ------------------------------------------------------------------ */
+typedef struct {
+  int kind;
+  uint32_t n_images, nch;
+  const void *pixels;          /* packed images (ImgType elements)  */
+  const uint64_t *img_off;     /* element offset of each image      */
+  const uint32_t *widths, *heights;
+  const uint8_t *labels;       /* packed label images (w*h each)    */
+  const uint64_t *lab_off;
+  const uint32_t *samples;     /* packed sample pixel indices       */
+  const uint64_t *samp_off;
+  const uint32_t *n_samples;
+} or_tset;
+
+typedef struct {
+  uint32_t nF, nT;
+  double flo[8], fhi[8];
+  double thrlo, thrhi;
+  uint32_t leafthr;
+} or_params;
+
+static size_t img_elem_size(int kind) { return kind == OR_FEAT_DEPTH_F32 ? 4 : 2; }
+
+static or_image image_of(const or_tset *ts, uint32_t i)
+{
+  or_image im;
+  im.data = (const char *)ts->pixels + ts->img_off[i] * img_elem_size(ts->kind);
+  im.w = (int)ts->widths[i];
+  im.h = (int)ts->heights[i];
+  im.nch = (int)ts->nch;
+  return im;
+}
+
+/* Tree arrays in the reference's SoA BFS layout (tree.hpp:72-173). */
+typedef struct {
+  int32_t *left;      /* -2 nonexistent, -1 leaf, 2i+1 split */
+  void *feats;        /* FeatType[N * FD] */
+  void *thrs;         /* FeatType[N] */
+  float *post;        /* float[N * nC] */
+} or_tree;
+
+/* Route one pixel from the root for `steps` levels; predict.cl:36-85
+ * applied once per level (cl_tree_trainer_impl_traverse_ts.hpp:229-243).
+ * A node whose leftChild is negative is a leaf (the reference reads out
+ * of bounds for an unsplit root with leftChild -2). */
+static int32_t route(int kind, const or_tree *tr, const or_image *im, int x, int y,
+                     uint32_t steps)
+{
+  int fd = feat_dim_of(kind);
+  size_t fsz = feat_is_short(kind) ? 2 : 4;
+  int32_t n = 0;
+  for (uint32_t s = 0; s < steps; s++) {
+    int32_t lc = tr->left[n];
+    if (lc < 0) break;
+    const void *p = (const char *)tr->feats + (size_t)n * fd * fsz;
+    double v = or_feature(kind, im, x, y, p);
+    n = lc + (goes_left(kind, v, tr->thrs, (uint32_t)n) ? 0 : 1);
+  }
+  return n;
+}
+
+/* ------------------------------------------------------------------ */
+/* A6 + A7: brute-force histogram H[slot][c][t][f] of the given nodes. */
+/* For each sample in node n (n listed in `nodes`), each candidate f    */
+/* and threshold t: H += (feat <= thr).  hist_update.cl:118-186 and    */
+/* cl_tree_trainer_impl_traverse_ts.hpp:518-617.  `sample_node` gives  */
+/* the node of every sample (packed order).  OpenMP over candidates.   */
+/* ------------------------------------------------------------------ */
+int or_hist_nodes(const or_tset *ts, const or_params *pr, uint32_t nC, uint32_t tree_id,
+                  uint32_t n_nodes, const int32_t *nodes, const int32_t *sample_node,
+                  uint32_t *H)
+{
+  int kind = ts->kind, fd = feat_dim_of(kind);
+  size_t fsz = feat_is_short(kind) ? 2 : 4;
+  uint32_t nF = pr->nF, nT = pr->nT;
+  size_t per = (size_t)nC * nT * nF;
+  memset(H, 0, sizeof(uint32_t) * per * n_nodes);
+  /* node id -> slot (linear search table over the listed nodes) */
+  int32_t maxn = 0;
+  for (uint32_t i = 0; i < n_nodes; i++) if (nodes[i] > maxn) maxn = nodes[i];
+  int32_t *slot = (int32_t *)malloc(sizeof(int32_t) * (size_t)(maxn + 1));
+  for (int32_t i = 0; i <= maxn; i++) slot[i] = -1;
+  for (uint32_t i = 0; i < n_nodes; i++) slot[nodes[i]] = (int32_t)i;
+
+  #pragma omp parallel for schedule(dynamic, 1)
+  for (int64_t fi = 0; fi < (int64_t)nF * n_nodes; fi++) {
+    uint32_t sl = (uint32_t)(fi / nF), f = (uint32_t)(fi % nF);
+    uint32_t node = (uint32_t)nodes[sl];
+    unsigned char params[64], thr[512];
+    or_feature_params(kind, tree_id, node, f, pr->flo, pr->fhi, params);
+    or_thresholds(kind, tree_id, node, f, nT, pr->thrlo, pr->thrhi, thr);
+    uint32_t *Hs = H + per * sl;
+    size_t g = 0;
+    for (uint32_t i = 0; i < ts->n_images; i++) {
+      or_image im = image_of(ts, i);
+      const uint8_t *lab = ts->labels + ts->lab_off[i];
+      const uint32_t *smp = ts->samples + ts->samp_off[i];
+      for (uint32_t s = 0; s < ts->n_samples[i]; s++, g++) {
+        if (sample_node[g] != (int32_t)node) continue;
+        uint32_t px = smp[s];
+        int x = (int)(px % (uint32_t)im.w), y = (int)(px / (uint32_t)im.w);
+        uint32_t c = (uint32_t)lab[px] - 1u;
+        double v = or_feature(kind, &im, x, y, params);
+        for (uint32_t t = 0; t < nT; t++)
+          Hs[(size_t)c * nT * nF + (size_t)t * nF + f] += (uint32_t)goes_left(kind, v, thr, t);
+      }
+    }
+  }
+  (void)fd; (void)fsz;
+  free(slot);
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Whole-tree training: CLTreeTrainer::train (cl_tree_trainer_impl.hpp */
+/* :203-255) with _initTrain (_impl_init.hpp:24-328), _initFrontier    */
+/* (:333-365), _traverseTrainingSet + consumer (_traverse_ts.hpp) and  */
+/* _learnBestFeatThr (_learn_best_featthr.hpp:25-283).  Samples are    */
+/* re-routed from the root at every level as the reference does.       */
+/* Per-node outputs (if non-NULL): chosen k, FP32 gain, FP64 gain.     */
+/* ------------------------------------------------------------------ */
+int or_train_tree(const or_tset *ts, const or_params *pr, uint32_t nC, const float *root_prior,
+                  uint32_t tree_id, uint32_t depth, uint32_t end_depth,
+                  int32_t *left, void *feats, void *thrs, float *post,
+                  uint32_t *node_k, float *node_gain, double *node_gain64)
+{
+  int kind = ts->kind, fd = feat_dim_of(kind);
+  size_t fsz = feat_is_short(kind) ? 2 : 4;
+  if (end_depth > depth || end_depth < 1) return -1;
+  size_t N = ((size_t)1 << depth) - 1;
+  uint32_t nF = pr->nF, nT = pr->nT;
+  /* Tree::_init (tree_impl.hpp:239-273) */
+  for (size_t i = 0; i < N; i++) left[i] = -2;
+  memset(feats, 0, N * fd * fsz);
+  memset(thrs, 0, N * fsz);
+  memset(post, 0, N * nC * sizeof(float));
+  memcpy(post, root_prior, nC * sizeof(float));   /* _impl_init.hpp:309-313 */
+
+  uint32_t *nodeTot = (uint32_t *)calloc(N, sizeof(uint32_t));
+  uint32_t *classTot = (uint32_t *)calloc(N * nC, sizeof(uint32_t));
+  size_t S = 0;
+  for (uint32_t i = 0; i < ts->n_images; i++) S += ts->n_samples[i];
+  nodeTot[0] = (uint32_t)S;                          /* _impl_init.hpp:68-83 */
+  int32_t *snode = (int32_t *)malloc(sizeof(int32_t) * (S ? S : 1));
+  unsigned char *infront = (unsigned char *)calloc(N, 1);
+  or_tree tr = {left, feats, thrs, post};
+
+  for (uint32_t d = 1; d < end_depth; d++) {
+    /* frontier */
+    size_t lvl = (d > 1) ? ((size_t)1 << (d - 1)) : 1;
+    size_t start = lvl - 1;
+    int32_t *front = (int32_t *)malloc(sizeof(int32_t) * lvl);
+    uint32_t nfront = 0;
+    if (d == 1) front[nfront++] = 0;
+    else
+      for (size_t i = 0; i < lvl; i++)
+        if (left[start + i] == -1 && nodeTot[start + i] > pr->leafthr) front[nfront++] = (int32_t)(start + i);
+    for (uint32_t q = 0; q < nfront; q++) infront[front[q]] = 1;
+    /* route every sample from the root for d-1 levels */
+    size_t g = 0;
+    for (uint32_t i = 0; i < ts->n_images; i++) {
+      or_image im = image_of(ts, i);
+      const uint8_t *lab = ts->labels + ts->lab_off[i];
+      const uint32_t *smp = ts->samples + ts->samp_off[i];
+      for (uint32_t s = 0; s < ts->n_samples[i]; s++, g++) {
+        uint32_t px = smp[s];
+        int x = (int)(px % (uint32_t)im.w), y = (int)(px / (uint32_t)im.w);
+        int32_t n = route(kind, &tr, &im, x, y, d - 1);
+        if (d == 1) classTot[(size_t)n * nC + (lab[px] - 1u)]++;   /* _traverse_ts.hpp:526 */
+        /* consumer gate, _traverse_ts.hpp:530-531 */
+        snode[g] = (infront[n] && nodeTot[n] > pr->leafthr) ? n : -1;
+      }
+    }
+    for (uint32_t q = 0; q < nfront; q++) infront[front[q]] = 0;
+    if (nfront == 0) { free(front); continue; }
+    size_t per = (size_t)nC * nT * nF;
+    uint32_t *H = (uint32_t *)malloc(sizeof(uint32_t) * per * nfront);
+    or_hist_nodes(ts, pr, nC, tree_id, nfront, front, snode, H);
+
+    for (uint32_t q = 0; q < nfront; q++) {
+      uint32_t n = (uint32_t)front[q];
+      uint32_t *Hn = H + per * q;
+      uint32_t *tot = classTot + (size_t)n * nC;
+      float g32;
+      uint32_t k = or_best_split(Hn, tot, nC, nT, nF, &g32);
+      uint32_t bt = k / nF, bf = k % nF;
+      uint32_t lh[256], rh[256], lS = 0, rS = 0;
+      for (uint32_t c = 0; c < nC; c++) {
+        lh[c] = Hn[(size_t)c * nT * nF + (size_t)bt * nF + bf];
+        rh[c] = tot[c] - lh[c];
+        lS += lh[c]; rS += rh[c];
+      }
+      if (node_k) node_k[n] = k;
+      if (node_gain) node_gain[n] = g32;
+      if (node_gain64) node_gain64[n] = or_gain_f64(lh, tot, nC);
+      if (!lS || !rS) continue;                       /* :145 */
+      nodeTot[2 * n + 1] = lS; nodeTot[2 * n + 2] = rS;
+      for (uint32_t c = 0; c < nC; c++) {
+        classTot[(size_t)(2 * n + 1) * nC + c] = lh[c];
+        classTot[(size_t)(2 * n + 2) * nC + c] = rh[c];
+        post[(size_t)(2 * n + 1) * nC + c] = ((float)lh[c]) / (float)lS;   /* :160-164 */
+        post[(size_t)(2 * n + 2) * nC + c] = ((float)rh[c]) / (float)rS;
+      }
+      left[2 * n + 1] = -1; left[2 * n + 2] = -1;
+      or_feature_params(kind, tree_id, n, bf, pr->flo, pr->fhi, (char *)feats + (size_t)n * fd * fsz);
+      unsigned char th[512];
+      or_thresholds(kind, tree_id, n, bf, bt + 1, pr->thrlo, pr->thrhi, th);
+      memcpy((char *)thrs + (size_t)n * fsz, th + (size_t)bt * fsz, fsz);
+      left[n] = (int32_t)(2 * n + 1);                 /* :233 */
+    }
+    free(H);
+    free(front);
+  }
+  free(infront);
+  free(snode);
+  free(classTot);
+  free(nodeTot);
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* A13: per-image priors (training_set_image_impl.hpp:35-52) and the  */
+/* training-set running mean (training_set_impl.hpp:157-170).          */
+/* ------------------------------------------------------------------ */
+void or_image_priors(const uint8_t *labels, const uint32_t *samples, uint32_t n_samples,
+                     uint32_t nC, float *out)
+{
+  uint32_t cnt[256] = {0}, tot = 0;
+  for (uint32_t i = 0; i < n_samples; i++) {
+    uint8_t l = labels[samples[i]];
+    if (l) { cnt[l - 1]++; tot++; }
+  }
+  for (uint32_t c = 0; c < nC; c++) out[c] = (float)cnt[c] / (float)tot;
+}
+
+void or_tset_priors(const or_tset *ts, uint32_t nC, float *out)
+{
+  float img[256];
+  for (uint32_t c = 0; c < nC; c++) out[c] = 0.0f;
+  for (uint32_t i = 0; i < ts->n_images; i++) {
+    or_image_priors(ts->labels + ts->lab_off[i], ts->samples + ts->samp_off[i],
+                    ts->n_samples[i], nC, img);
+    for (uint32_t c = 0; c < nC; c++) {
+      double pC = out[c], pI = img[c];
+      out[c] = (float)(pC * i / (i + 1) + pI / (i + 1));
+    }
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* A14/A15: prediction.  Leaf index per pixel for one tree            */
+/* (cl_classifier_impl.hpp:212-341), masked-out pixels get -1; and the */
+/* forest posterior: planar [c][y][x], FP32 sum over trees in order,   */
+/* then divided by nTrees (cl_classifier_impl.hpp:478-552).            */
+/* ------------------------------------------------------------------ */
+void or_predict_leaves(int kind, const void *img, int w, int h, int nch, const uint8_t *mask,
+                       const int32_t *left, const void *feats, const void *thrs, uint32_t depth,
+                       int32_t *out)
+{
+  or_image im = {img, w, h, nch};
+  or_tree tr = {(int32_t *)left, (void *)feats, (void *)thrs, NULL};
+  for (int y = 0; y < h; y++)
+    for (int x = 0; x < w; x++) {
+      size_t p = (size_t)y * w + x;
+      out[p] = (mask && !mask[p]) ? -1 : route(kind, &tr, &im, x, y, depth - 1);
+    }
+}
+
+void or_predict_posterior(int kind, const void *img, int w, int h, int nch, const uint8_t *mask,
+                          uint32_t n_trees, const int32_t *const *lefts, const void *const *feats,
+                          const void *const *thrs, const float *const *posts, const uint32_t *depths,
+                          uint32_t nC, float *out)
+{
+  or_image im = {img, w, h, nch};
+  size_t P = (size_t)w * h;
+  memset(out, 0, sizeof(float) * P * nC);
+  for (uint32_t t = 0; t < n_trees; t++) {
+    or_tree tr = {(int32_t *)lefts[t], (void *)feats[t], (void *)thrs[t], NULL};
+    for (int y = 0; y < h; y++)
+      for (int x = 0; x < w; x++) {
+        size_t p = (size_t)y * w + x;
+        if (mask && !mask[p]) continue;
+        int32_t n = route(kind, &tr, &im, x, y, depths[t] - 1);
+        for (uint32_t c = 0; c < nC; c++) out[c * P + p] += posts[t][(size_t)n * nC + c];
+      }
+  }
+  for (size_t i = 0; i < P * nC; i++) out[i] /= (float)n_trees;
+}
+
+int or_num_threads(void)
+{
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}

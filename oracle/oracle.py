@@ -1,0 +1,247 @@
+"""ctypes binding of the CPU oracle (oracle/oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and
+by the cpu_baseline leg of bench.py, always as the checker or the CPU
+baseline, never as the product path.  Build: ``make -C oracle``.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_build", "liboracle.so")
+REF_PRNG = os.path.join(HERE, "_ref", "libref_prng.so")
+
+KIND_DEPTH_F32, KIND_DEPTH_U16, KIND_RGBD, KIND_DEMO_S16 = 0, 1, 2, 3
+FD = {0: 4, 1: 4, 2: 6, 3: 2}
+FDT = {0: np.float32, 1: np.float32, 2: np.float32, 3: np.int16}
+IMGT = {0: np.float32, 1: np.uint16, 2: np.uint16, 3: np.uint16}
+NCH = {0: 1, 1: 1, 2: 4, 3: 1}
+
+VP = C.c_void_p
+
+
+class OrTset(C.Structure):
+    _fields_ = [("kind", C.c_int), ("n_images", C.c_uint32), ("nch", C.c_uint32), ("pixels", VP),
+                ("img_off", VP), ("widths", VP), ("heights", VP), ("labels", VP), ("lab_off", VP),
+                ("samples", VP), ("samp_off", VP), ("n_samples", VP)]
+
+
+class OrParams(C.Structure):
+    _fields_ = [("nF", C.c_uint32), ("nT", C.c_uint32), ("flo", C.c_double * 8), ("fhi", C.c_double * 8),
+                ("thrlo", C.c_double), ("thrhi", C.c_double), ("leafthr", C.c_uint32)]
+
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        L.or_md5rand.argtypes = [VP, VP]
+        L.or_sample_f32.restype = C.c_float
+        L.or_sample_f32.argtypes = [C.c_uint32, C.c_float, C.c_float]
+        L.or_sample_s16.restype = C.c_int16
+        L.or_sample_s16.argtypes = [C.c_uint32, C.c_int16, C.c_int16]
+        L.or_feature_params.argtypes = [C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, VP, VP, VP]
+        L.or_thresholds.argtypes = [C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_double,
+                                    C.c_double, VP]
+        L.or_feature_at.restype = C.c_double
+        L.or_feature_at.argtypes = [C.c_int, VP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, VP]
+        L.or_log2_det.restype = C.c_float
+        L.or_log2_det.argtypes = [C.c_float]
+        L.or_gain_f32.restype = C.c_float
+        L.or_gain_f32.argtypes = [VP, VP, C.c_uint32]
+        L.or_gain_f64.restype = C.c_double
+        L.or_gain_f64.argtypes = [VP, VP, C.c_uint32]
+        L.or_best_split.restype = C.c_uint32
+        L.or_best_split.argtypes = [VP, VP, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(C.c_float)]
+        L.or_hist_nodes.argtypes = [C.POINTER(OrTset), C.POINTER(OrParams), C.c_uint32, C.c_uint32, C.c_uint32,
+                                    VP, VP, VP]
+        L.or_train_tree.argtypes = [C.POINTER(OrTset), C.POINTER(OrParams), C.c_uint32, VP, C.c_uint32,
+                                    C.c_uint32, C.c_uint32, VP, VP, VP, VP, VP, VP, VP]
+        L.or_tset_priors.argtypes = [C.POINTER(OrTset), C.c_uint32, VP]
+        L.or_predict_leaves.argtypes = [C.c_int, VP, C.c_int, C.c_int, C.c_int, VP, VP, VP, VP, C.c_uint32, VP]
+        L.or_predict_posterior.argtypes = [C.c_int, VP, C.c_int, C.c_int, C.c_int, VP, C.c_uint32, VP, VP, VP,
+                                           VP, VP, C.c_uint32, VP]
+        L.or_num_threads.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def md5rand(seed):
+    s = np.ascontiguousarray(seed, dtype=np.uint32)
+    out = np.zeros(4, dtype=np.uint32)
+    lib().or_md5rand(_p(s), _p(out))
+    return out
+
+
+def ref_md5rand():
+    """The reference's own md5Rand (src/padenti/prng.hpp) when oracle/_ref was built."""
+    if not os.path.exists(REF_PRNG):
+        return None
+    L = C.CDLL(REF_PRNG)
+    L.ref_md5rand.argtypes = [VP, VP]
+
+    def f(seed):
+        s = np.ascontiguousarray(seed, dtype=np.uint32)
+        out = np.zeros(4, dtype=np.uint32)
+        L.ref_md5rand(_p(s), _p(out))
+        return out
+    return f
+
+
+def feature_params(kind, tree, node, f, lo, hi):
+    lo8 = np.zeros(8)
+    hi8 = np.zeros(8)
+    lo8[:len(lo)] = lo
+    hi8[:len(hi)] = hi
+    out = np.zeros(FD[kind], dtype=FDT[kind])
+    lib().or_feature_params(kind, tree, node, f, _p(lo8), _p(hi8), _p(out))
+    return out
+
+
+def thresholds(kind, tree, node, f, nT, lo, hi):
+    out = np.zeros(nT, dtype=FDT[kind])
+    lib().or_thresholds(kind, tree, node, f, nT, lo, hi, _p(out))
+    return out
+
+
+def feature_at(kind, image, x, y, params):
+    img = np.ascontiguousarray(image, dtype=IMGT[kind])
+    h, w = img.shape[0], img.shape[1]
+    p = np.ascontiguousarray(params, dtype=FDT[kind])
+    return lib().or_feature_at(kind, _p(img), w, h, NCH[kind], x, y, _p(p))
+
+
+def log2_det(x):
+    return np.float32(lib().or_log2_det(float(x)))
+
+
+def gain_f32(left, tot):
+    l_ = np.ascontiguousarray(left, dtype=np.uint32)
+    t_ = np.ascontiguousarray(tot, dtype=np.uint32)
+    return np.float32(lib().or_gain_f32(_p(l_), _p(t_), len(t_)))
+
+
+def gain_f64(left, tot):
+    l_ = np.ascontiguousarray(left, dtype=np.uint32)
+    t_ = np.ascontiguousarray(tot, dtype=np.uint32)
+    return lib().or_gain_f64(_p(l_), _p(t_), len(t_))
+
+
+def best_split(H, tot):
+    """H[c][t][f] (uint32), tot[c] -> (k, gain)."""
+    H = np.ascontiguousarray(H, dtype=np.uint32)
+    t_ = np.ascontiguousarray(tot, dtype=np.uint32)
+    g = C.c_float()
+    nC, nT, nF = H.shape
+    k = lib().or_best_split(_p(H), _p(t_), nC, nT, nF, C.byref(g))
+    return int(k), np.float32(g.value)
+
+
+class Tset:
+    """Oracle view of a padenti_amd.TrainingSet (keeps the packed arrays alive)."""
+
+    def __init__(self, kind, ts):
+        (self.pix, self.img_off, self.w, self.h, self.labels, self.lab_off, self.samples, self.samp_off,
+         self.ns) = ts.packed()
+        self.pix = np.ascontiguousarray(self.pix, dtype=IMGT[kind])
+        self.c = OrTset(kind, len(ts.images), NCH[kind], _p(self.pix), _p(self.img_off), _p(self.w), _p(self.h),
+                        _p(self.labels), _p(self.lab_off), _p(self.samples), _p(self.samp_off), _p(self.ns))
+        self.kind = kind
+        self.n_classes = ts.n_classes
+        self.total = int(self.ns.sum())
+
+
+def params(p):
+    """OrParams from a padenti_amd.TreeTrainerParameters."""
+    o = OrParams()
+    o.nF, o.nT = p.n_features, p.n_thresholds
+    for i, v in enumerate(p.feat_low_bounds):
+        o.flo[i] = float(v)
+    for i, v in enumerate(p.feat_up_bounds):
+        o.fhi[i] = float(v)
+    o.thrlo, o.thrhi, o.leafthr = float(p.thr_low_bound), float(p.thr_up_bound), p.per_leaf_samples_thr
+    return o
+
+
+def hist_nodes(tset, p, tree_id, nodes, sample_node):
+    nodes = np.ascontiguousarray(nodes, dtype=np.int32)
+    sn = np.ascontiguousarray(sample_node, dtype=np.int32)
+    H = np.zeros((len(nodes), tset.n_classes, p.n_thresholds, p.n_features), dtype=np.uint32)
+    lib().or_hist_nodes(C.byref(tset.c), C.byref(params(p)), tset.n_classes, tree_id, len(nodes), _p(nodes),
+                        _p(sn), _p(H))
+    return H
+
+
+def train_tree(tset, p, prior, tree_id, depth, end_depth=None):
+    end_depth = end_depth or depth
+    n = (1 << depth) - 1
+    fd, fdt, nC = FD[tset.kind], FDT[tset.kind], tset.n_classes
+    out = {
+        "left_children": np.zeros(n, dtype=np.int32),
+        "features": np.zeros((n, fd), dtype=fdt),
+        "thresholds": np.zeros(n, dtype=fdt),
+        "posteriors": np.zeros((n, nC), dtype=np.float32),
+        "node_k": np.zeros(n, dtype=np.uint32),
+        "node_gain": np.zeros(n, dtype=np.float32),
+        "node_gain64": np.zeros(n, dtype=np.float64),
+    }
+    pr = np.ascontiguousarray(prior, dtype=np.float32)
+    rc = lib().or_train_tree(C.byref(tset.c), C.byref(params(p)), nC, _p(pr), tree_id, depth, end_depth,
+                             _p(out["left_children"]), _p(out["features"]), _p(out["thresholds"]),
+                             _p(out["posteriors"]), _p(out["node_k"]), _p(out["node_gain"]),
+                             _p(out["node_gain64"]))
+    if rc != 0:
+        raise RuntimeError("oracle training failed")
+    return out
+
+
+def tset_priors(tset):
+    out = np.zeros(tset.n_classes, dtype=np.float32)
+    lib().or_tset_priors(C.byref(tset.c), tset.n_classes, _p(out))
+    return out
+
+
+def predict_leaves(kind, image, mask, tree):
+    img = np.ascontiguousarray(image, dtype=IMGT[kind])
+    h, w = img.shape[0], img.shape[1]
+    out = np.zeros(w * h, dtype=np.int32)
+    m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
+    lib().or_predict_leaves(kind, _p(img), w, h, NCH[kind], _p(m), _p(tree.left_children), _p(tree.features),
+                            _p(tree.thresholds), tree.depth, _p(out))
+    return out.reshape(h, w)
+
+
+def predict_posterior(kind, image, mask, trees, n_classes):
+    img = np.ascontiguousarray(image, dtype=IMGT[kind])
+    h, w = img.shape[0], img.shape[1]
+    out = np.zeros(n_classes * w * h, dtype=np.float32)
+    m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
+    nt = len(trees)
+    lefts = (VP * nt)(*[_p(t.left_children) for t in trees])
+    feats = (VP * nt)(*[_p(t.features) for t in trees])
+    thrs = (VP * nt)(*[_p(t.thresholds) for t in trees])
+    posts = (VP * nt)(*[_p(t.posteriors) for t in trees])
+    depths = np.array([t.depth for t in trees], dtype=np.uint32)
+    lib().or_predict_posterior(kind, _p(img), w, h, NCH[kind], _p(m), nt, lefts, feats, thrs, posts, _p(depths),
+                               n_classes, _p(out))
+    return out.reshape(n_classes, h, w)
+
+
+def num_threads():
+    return lib().or_num_threads()

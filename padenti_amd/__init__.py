@@ -1,0 +1,231 @@
+"""Padenti per-pixel random-forest hot path on MI355X (gfx950).
+
+Host-side mirror of the reference C++ API (src/padenti/*.hpp) over the C ABI
+of ``libpadenti_hip.so``:
+
+  TrainingSetImage / TrainingSet   training_set_image.hpp:38-110, training_set.hpp:39-112
+  Tree                             tree.hpp:72-173 (SoA, breadth-first heap)
+  TreeTrainerParameters            tree_trainer.hpp:35-51
+  HIPTreeTrainer                   CLTreeTrainer (cl_tree_trainer.hpp:115, train :203-255)
+  HIPClassifier                    CLClassifier (cl_classifier.hpp:81-94, predict x4)
+
+The C++ drop-in headers with the same names live in include/padenti/.
+Everything computes on the GPU; nothing here falls back to the CPU.
+"""
+import numpy as np
+
+from . import _lib as L
+from ._lib import (PD_FEAT_DEMO_S16, PD_FEAT_DEPTH_F32, PD_FEAT_DEPTH_U16, PD_FEAT_RGBD_U16X4,
+                   PD_MODE_LEVEL_BATCHED, PD_MODE_PER_NODE, FEATURES, PdError, check, lib, ptr)
+
+__all__ = [
+    "TrainingSetImage", "TrainingSet", "Tree", "TreeTrainerParameters", "HIPTreeTrainer",
+    "HIPClassifier", "PdError", "PD_FEAT_DEPTH_F32", "PD_FEAT_DEPTH_U16", "PD_FEAT_RGBD_U16X4",
+    "PD_FEAT_DEMO_S16", "PD_MODE_PER_NODE", "PD_MODE_LEVEL_BATCHED", "FEATURES", "device_count",
+]
+
+
+def device_count():
+    return lib().pd_device_count()
+
+
+class TrainingSetImage:
+    """Image + labels (uint8, 0 = unlabelled) + sampled pixel indices.
+
+    Per-image priors follow training_set_image_impl.hpp:35-52: class
+    frequencies over the labelled sampled pixels, as float32.
+    """
+
+    def __init__(self, data, width, height, labels, n_classes, samples):
+        self.data = np.ascontiguousarray(data)
+        self.width, self.height = int(width), int(height)
+        self.labels = np.ascontiguousarray(labels, dtype=np.uint8).reshape(-1)
+        self.samples = np.ascontiguousarray(samples, dtype=np.uint32).reshape(-1)
+        self.n_classes = int(n_classes)
+        lab = self.labels[self.samples]
+        lab = lab[lab > 0]
+        cnt = np.bincount(lab.astype(np.int64) - 1, minlength=n_classes)[:n_classes].astype(np.float32)
+        self.priors = (cnt / np.float32(len(lab))).astype(np.float32)
+
+
+class TrainingSet:
+    """Ordered image list with the running-mean priors of
+    training_set_impl.hpp:157-170 (p <- (float)(p*n/(n+1) + p_img/(n+1)) in FP64)."""
+
+    def __init__(self, n_classes):
+        self.n_classes = int(n_classes)
+        self.images = []
+        self.priors = np.zeros(n_classes, dtype=np.float32)
+
+    def __lshift__(self, img):
+        n = len(self.images)
+        self.images.append(img)
+        pc = self.priors.astype(np.float64)
+        pi = img.priors.astype(np.float64)
+        self.priors = (pc * n / (n + 1) + pi / (n + 1)).astype(np.float32)
+        return self
+
+    def packed(self):
+        """(pixels, img_off, widths, heights, labels, lab_off, samples, samp_off, n_samples)."""
+        imgs = self.images
+        pix = np.concatenate([im.data.reshape(-1) for im in imgs])
+        sizes = np.array([im.data.size for im in imgs], dtype=np.uint64)
+        img_off = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.uint64)
+        labels = np.concatenate([im.labels for im in imgs])
+        lsz = np.array([im.labels.size for im in imgs], dtype=np.uint64)
+        lab_off = np.concatenate([[0], np.cumsum(lsz)[:-1]]).astype(np.uint64)
+        samples = np.concatenate([im.samples for im in imgs]).astype(np.uint32)
+        ns = np.array([im.samples.size for im in imgs], dtype=np.uint32)
+        samp_off = np.concatenate([[0], np.cumsum(ns.astype(np.uint64))[:-1]]).astype(np.uint64)
+        w = np.array([im.width for im in imgs], dtype=np.uint32)
+        h = np.array([im.height for im in imgs], dtype=np.uint32)
+        return pix, img_off, w, h, labels, lab_off, samples, samp_off, ns
+
+
+class Tree:
+    """Breadth-first SoA tree of depth D: 2^D - 1 nodes (tree_impl.hpp:239-273).
+    left_children: -2 nonexistent, -1 leaf, 2i+1 split."""
+
+    def __init__(self, tree_id, depth, feature_kind, n_classes):
+        _, _, fdt, fd = FEATURES[feature_kind]
+        self.id, self.depth, self.n_classes, self.kind = int(tree_id), int(depth), int(n_classes), feature_kind
+        n = (1 << depth) - 1
+        self.left_children = np.full(n, -2, dtype=np.int32)
+        self.features = np.zeros((n, fd), dtype=fdt)
+        self.thresholds = np.zeros(n, dtype=fdt)
+        self.posteriors = np.zeros((n, n_classes), dtype=np.float32)
+
+    @property
+    def n_nodes(self):
+        return self.left_children.size
+
+
+class TreeTrainerParameters:
+    """tree_trainer.hpp:35-51 (computeFRange / randomThrSampling unused there too)."""
+
+    def __init__(self, n_features, n_thresholds, feat_low_bounds, feat_up_bounds, thr_low_bound,
+                 thr_up_bound, per_leaf_samples_thr):
+        self.n_features = int(n_features)
+        self.n_thresholds = int(n_thresholds)
+        self.feat_low_bounds = list(feat_low_bounds)
+        self.feat_up_bounds = list(feat_up_bounds)
+        self.thr_low_bound = thr_low_bound
+        self.thr_up_bound = thr_up_bound
+        self.per_leaf_samples_thr = int(per_leaf_samples_thr)
+
+    def c(self):
+        return L.make_params(self.n_features, self.n_thresholds, self.feat_low_bounds, self.feat_up_bounds,
+                             self.thr_low_bound, self.thr_up_bound, self.per_leaf_samples_thr)
+
+
+class HIPTreeTrainer:
+    """Replaces CLTreeTrainer(featurePath, useCPU): the feature is one of the
+    compiled HIP hooks, selected by ``feature_kind``."""
+
+    def __init__(self, feature_kind, n_classes, device=0):
+        import ctypes
+        self.kind, self.n_classes, self.device = feature_kind, int(n_classes), device
+        h = ctypes.c_void_p()
+        check(lib().pd_trainer_create(ctypes.byref(h), device, feature_kind, n_classes))
+        self._h = h
+        self._ts = None
+        self.last_level_stats = []
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pd_trainer_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_training_set(self, ts):
+        """Upload the training set once; later train() calls reuse it."""
+        pix, img_off, w, h, labels, lab_off, samples, samp_off, ns = ts.packed()
+        want = FEATURES[self.kind][0]
+        pix = np.ascontiguousarray(pix, dtype=want)
+        check(lib().pd_trainer_set_data(self._h, len(ts.images), ptr(pix), ptr(img_off), ptr(w), ptr(h),
+                                        ptr(labels), ptr(lab_off), ptr(samples), ptr(samp_off), ptr(ns)))
+        self._ts = ts
+
+    def set_hist_budget(self, nbytes):
+        check(lib().pd_trainer_set_hist_budget(self._h, int(nbytes)))
+
+    def set_chunk(self, samples_per_chunk):
+        check(lib().pd_trainer_set_chunk(self._h, int(samples_per_chunk)))
+
+    def train(self, tree, ts, params, start_depth, end_depth, mode=PD_MODE_LEVEL_BATCHED, diagnostics=False):
+        """TreeTrainer::train(tree, ts, params, startDepth=1, endDepth)."""
+        if start_depth != 1:
+            raise ValueError("Starting depth must be equal to 1")   # cl_tree_trainer_impl.hpp:210
+        if ts is not self._ts:
+            self.set_training_set(ts)
+        n = tree.n_nodes
+        nk = np.zeros(n, dtype=np.uint32) if diagnostics else None
+        ng = np.zeros(n, dtype=np.float32) if diagnostics else None
+        ng64 = np.zeros(n, dtype=np.float64) if diagnostics else None
+        stats = (L.LevelStats * max(end_depth - 1, 1))()
+        prior = np.ascontiguousarray(ts.priors, dtype=np.float32)
+        p = params.c()
+        check(lib().pd_trainer_train(self._h, p, tree.id, tree.depth, end_depth, mode, ptr(prior),
+                                     ptr(tree.left_children), ptr(tree.features), ptr(tree.thresholds),
+                                     ptr(tree.posteriors), ptr(nk), ptr(ng), ptr(ng64), stats))
+        self.last_level_stats = [stats[i].as_dict() for i in range(max(end_depth - 1, 0))]
+        if diagnostics:
+            return {"node_k": nk, "node_gain": ng, "node_gain64": ng64}
+        return None
+
+    def hist_nodes(self, params, tree_id, nodes, sample_node):
+        nodes = np.ascontiguousarray(nodes, dtype=np.int32)
+        sample_node = np.ascontiguousarray(sample_node, dtype=np.int32)
+        H = np.zeros((len(nodes), self.n_classes, params.n_thresholds, params.n_features), dtype=np.uint32)
+        check(lib().pd_trainer_hist_nodes(self._h, params.c(), tree_id, len(nodes), ptr(nodes), ptr(sample_node),
+                                          ptr(H)))
+        return H
+
+
+class HIPClassifier:
+    """Replaces CLClassifier: ``clf << tree`` uploads a tree; predict_leaves /
+    predict_posterior are the reference's four predict overloads
+    (classifier.hpp:52-94; mask=None means every pixel)."""
+
+    def __init__(self, feature_kind, n_classes, device=0):
+        import ctypes
+        self.kind, self.n_classes, self.device = feature_kind, int(n_classes), device
+        h = ctypes.c_void_p()
+        check(lib().pd_forest_create(ctypes.byref(h), device, feature_kind, n_classes))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pd_forest_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __lshift__(self, tree):
+        check(lib().pd_forest_add_tree(self._h, tree.depth, ptr(tree.left_children), ptr(tree.features),
+                                       ptr(tree.thresholds), ptr(tree.posteriors)))
+        return self
+
+    @property
+    def n_trees(self):
+        return lib().pd_forest_n_trees(self._h)
+
+    def _img(self, image):
+        return np.ascontiguousarray(image, dtype=FEATURES[self.kind][0])
+
+    def predict_leaves(self, tree_index, image, width, height, mask=None):
+        img = self._img(image)
+        out = np.empty(width * height, dtype=np.int32)
+        m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8).reshape(-1)
+        check(lib().pd_forest_predict_leaves(self._h, tree_index, ptr(img), width, height, ptr(m), ptr(out)))
+        return out.reshape(height, width)
+
+    def predict_posterior(self, image, width, height, mask=None):
+        img = self._img(image)
+        out = np.empty(self.n_classes * width * height, dtype=np.float32)
+        m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8).reshape(-1)
+        check(lib().pd_forest_predict_posterior(self._h, ptr(img), width, height, ptr(m), ptr(out)))
+        return out.reshape(self.n_classes, height, width)

@@ -1,0 +1,160 @@
+"""ctypes binding of libpadenti_hip.so (the C ABI in include/padenti_hip.h).
+
+The library is built in-tree (``python -m padenti_amd.build``).  Loading
+fails loudly when it is missing: there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpadenti_hip.so")
+
+PD_OK = 0
+PD_FEAT_DEPTH_F32 = 0
+PD_FEAT_DEPTH_U16 = 1
+PD_FEAT_RGBD_U16X4 = 2
+PD_FEAT_DEMO_S16 = 3
+PD_MODE_PER_NODE = 0
+PD_MODE_LEVEL_BATCHED = 1
+
+PD_SYN_BLOBS_F32 = 0
+PD_SYN_CAPSULES_U16 = 1
+PD_SYN_CAPSULES_RGBD = 2
+
+# (image dtype, channels, feature dtype, feature dim) of each compiled hook
+FEATURES = {
+    PD_FEAT_DEPTH_F32: (np.float32, 1, np.float32, 4),
+    PD_FEAT_DEPTH_U16: (np.uint16, 1, np.float32, 4),
+    PD_FEAT_RGBD_U16X4: (np.uint16, 4, np.float32, 6),
+    PD_FEAT_DEMO_S16: (np.uint16, 1, np.int16, 2),
+}
+
+
+class PdError(RuntimeError):
+    """A C-ABI call returned a negative status."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"padenti_hip error {code}: {msg}")
+        self.code = code
+
+
+class TrainParams(C.Structure):
+    _fields_ = [
+        ("n_features", C.c_uint32),
+        ("n_thresholds", C.c_uint32),
+        ("feat_low", C.c_double * 8),
+        ("feat_up", C.c_double * 8),
+        ("thr_low", C.c_double),
+        ("thr_up", C.c_double),
+        ("per_leaf_samples_thr", C.c_uint32),
+        ("reserved", C.c_uint32),
+    ]
+
+
+class LevelStats(C.Structure):
+    _fields_ = [
+        ("level", C.c_uint32),
+        ("frontier_nodes", C.c_uint32),
+        ("slices", C.c_uint32),
+        ("hist_launches", C.c_uint32),
+        ("active_samples", C.c_uint64),
+        ("evals", C.c_double),
+        ("hist_ms", C.c_double),
+        ("split_ms", C.c_double),
+        ("route_ms", C.c_double),
+        ("wall_ms", C.c_double),
+        ("split_nodes", C.c_uint32),
+        ("reserved", C.c_uint32),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+_lib = None
+
+VP = C.c_void_p
+U32 = C.c_uint32
+U64 = C.c_uint64
+I32 = C.c_int
+
+
+def lib():
+    """Load the shared library once (raises if it was never built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -m padenti_amd.build` "
+            "(the HIP path has no fallback)")
+    L = C.CDLL(LIB_PATH)
+    sig = {
+        "pd_last_error": (C.c_char_p, []),
+        "pd_version": (I32, []),
+        "pd_device_count": (I32, []),
+        "pd_trainer_create": (I32, [C.POINTER(VP), I32, I32, U32]),
+        "pd_trainer_destroy": (I32, [VP]),
+        "pd_trainer_set_data": (I32, [VP, U32, VP, VP, VP, VP, VP, VP, VP, VP, VP]),
+        "pd_trainer_set_hist_budget": (I32, [VP, U64]),
+        "pd_trainer_set_chunk": (I32, [VP, U32]),
+        "pd_trainer_train": (I32, [VP, C.POINTER(TrainParams), U32, U32, U32, I32, VP, VP, VP, VP, VP, VP,
+                                   VP, VP, VP]),
+        "pd_trainer_hist_nodes": (I32, [VP, C.POINTER(TrainParams), U32, U32, VP, VP, VP]),
+        "pd_split_search": (I32, [I32, U32, U32, U32, U32, VP, VP, VP, VP, VP]),
+        "pd_candidate_tables": (I32, [I32, I32, C.POINTER(TrainParams), U32, U32, VP, VP, VP]),
+        "pd_md5rand": (I32, [I32, U32, VP, VP]),
+        "pd_log2_det": (I32, [I32, U32, VP, VP]),
+        "pd_feature_eval": (I32, [I32, I32, VP, U32, U32, U32, VP, VP, VP]),
+        "pd_forest_create": (I32, [C.POINTER(VP), I32, I32, U32]),
+        "pd_forest_destroy": (I32, [VP]),
+        "pd_forest_add_tree": (I32, [VP, U32, VP, VP, VP, VP]),
+        "pd_forest_n_trees": (I32, [VP]),
+        "pd_forest_predict_leaves": (I32, [VP, U32, VP, U32, U32, VP, VP]),
+        "pd_forest_predict_posterior": (I32, [VP, VP, U32, U32, VP, VP]),
+        "pd_forest_predict_frames_dev": (I32, [VP, U32, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, I32,
+                                               C.POINTER(C.c_float)]),
+        "pd_dev_alloc": (I32, [I32, U64, C.POINTER(VP)]),
+        "pd_dev_free": (I32, [VP]),
+        "pd_memcpy_h2d": (I32, [VP, VP, U64]),
+        "pd_memcpy_d2h": (I32, [VP, VP, U64]),
+        "pd_dev_sync": (I32, [I32]),
+        "pd_synth_image": (I32, [I32, U64, U32, U32, U32, U32, VP, VP]),
+        "pd_synth_samples": (I32, [U64, U32, VP, U32, U32, U32, VP]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def check(code):
+    if code < 0:
+        raise PdError(code, lib().pd_last_error().decode())
+    return code
+
+
+def ptr(a):
+    """Raw pointer of a C-contiguous numpy array (None passes NULL)."""
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "array must be C-contiguous"
+    return a.ctypes.data
+
+
+def make_params(n_features, n_thresholds, feat_low, feat_up, thr_low, thr_up, per_leaf_samples_thr):
+    p = TrainParams()
+    p.n_features = n_features
+    p.n_thresholds = n_thresholds
+    for i, v in enumerate(feat_low):
+        p.feat_low[i] = float(v)
+    for i, v in enumerate(feat_up):
+        p.feat_up[i] = float(v)
+    p.thr_low = float(thr_low)
+    p.thr_up = float(thr_up)
+    p.per_leaf_samples_thr = per_leaf_samples_thr
+    return p

@@ -1,0 +1,521 @@
+// gfx950 kernels of the random-forest hot path.
+//
+//  k_hist      (a) per-node class histograms H[slot][c][t][f]
+//  k_split     (b) information-gain argmax with lowest-k tie-break
+//  k_finalize      winner decode + left class counts
+//  k_route         one tree level of sample routing + partition by child
+//  k_predict   (c) per-pixel forest traversal (leaf ids / FP32 posteriors)
+//
+// Histogram design (replaces hist_update.cl:36-187 + the host SSE2
+// accumulation cl_tree_trainer_impl_traverse_ts.hpp:518-617):
+//  * one work item = (frontier slot, chunk of that node's samples, tile of 64
+//    candidate features); a block of 4 wave64s owns the item;
+//  * lane l of every wave owns candidate f = 64*tile + l: its FeatDim
+//    parameters and nT thresholds are derived by MD5 once per block (as the
+//    reference derives them per (sample, feature)) and then held in VGPRs;
+//  * the waves stride over the chunk's samples; a sample is wave-uniform, so
+//    its record and image descriptor come through scalar loads;
+//  * for each (sample, candidate) the lane counts r = #{t : !(v <= thr_t)}
+//    and does ONE LDS add into rank bin [c][r] (16-bit halves of a u32);
+//  * at the end, count(c, t) = sum_{r <= L_t} bin[c][r] with
+//    L_t = #{t' : thr_t' < thr_t}, which equals #{samples : v <= thr_t}
+//    exactly (proof in the kernel comment), and is stored (single-chunk
+//    node) or atomically added (multi-chunk node) to H in the reference's
+//    [c][t][f] layout.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "../../include/padenti/pd_math.hpp"
+#include "../../include/padenti/pd_features.hpp"
+
+#pragma clang fp contract(off)
+
+namespace pd {
+
+struct ImgDesc {
+  uint64_t off;   // element offset of the image in the packed pixel buffer
+  uint32_t w, h;
+};
+
+struct HistItem {
+  uint32_t slot;    // slice-local frontier slot
+  uint32_t begin;   // first entry in the active sample list
+  uint32_t count;   // number of samples (<= 65535)
+  uint32_t ftile;   // candidate tile: features [64*ftile, 64*ftile+64)
+};
+
+template <class F>
+struct Bounds {
+  typename F::FeatType lo[F::FeatDim], hi[F::FeatDim];
+  typename F::FeatType tlo, thi;
+};
+
+// sample record: x = image | class << 24, y = px_x | px_y << 16
+__device__ __forceinline__ uint32_t rec_img(uint2 r) { return r.x & 0xffffffu; }
+__device__ __forceinline__ uint32_t rec_cls(uint2 r) { return r.x >> 24; }
+__device__ __forceinline__ int rec_x(uint2 r) { return (int)(r.y & 0xffffu); }
+__device__ __forceinline__ int rec_y(uint2 r) { return (int)(r.y >> 16); }
+
+constexpr int HIST_THREADS = 256;
+constexpr int HIST_WAVES = 4;
+
+// words per (class, lane) of the 16-bit rank bins for ranks 0..NTP
+template <int NTP>
+constexpr int rank_words() { return (NTP + 2) / 2; }
+
+template <class F, int NTP>
+constexpr size_t hist_lds_bytes(uint32_t nC)
+{
+  return (size_t)nC * rank_words<NTP>() * 64 * 4 +                         // rank bins
+         (size_t)(F::FeatDim + NTP) * 64 * sizeof(typename F::FeatType) +  // staging
+         (size_t)NTP * 64;                                                   // ranks L_t
+}
+
+template <class F, int NTP>
+__global__ __launch_bounds__(HIST_THREADS) void k_hist(
+    const typename F::ImgType *__restrict__ pixels, const ImgDesc *__restrict__ imgs,
+    const uint2 *__restrict__ recs, const HistItem *__restrict__ items, uint32_t item_base,
+    const int32_t *__restrict__ slot_node, const uint8_t *__restrict__ slot_atomic, Bounds<F> bnd,
+    uint32_t tree, uint32_t nF, uint32_t nT, uint32_t nC, uint32_t *__restrict__ H)
+{
+  using T = typename F::FeatType;
+  constexpr int FD = F::FeatDim;
+  constexpr int RW = rank_words<NTP>();
+  extern __shared__ __align__(16) uint32_t lds[];
+  uint32_t *bins = lds;
+  T *sp = (T *)(bins + (size_t)nC * RW * 64);
+  T *st = sp + FD * 64;
+  uint8_t *sl = (uint8_t *)(st + NTP * 64);
+
+  const HistItem it = items[item_base + blockIdx.x];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t lane = tid & 63u;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const uint32_t f = it.ftile * 64u + lane;
+  const bool fv = f < nF;
+  const uint32_t node = (uint32_t)slot_node[it.slot];
+
+  for (uint32_t i = tid; i < nC * RW * 64u; i += HIST_THREADS) bins[i] = 0u;
+
+  // candidate parameters (wave 0) and thresholds (wave 1), one MD5 chain each
+  if (tid < 64) {
+    T p[FD];
+    candidate_params<T, FD>(tree, node, f, bnd.lo, bnd.hi, p);
+#pragma unroll
+    for (int j = 0; j < FD; j++) sp[j * 64 + lane] = p[j];
+  } else if (tid < 128) {
+    U4 s = {tree, node, f, 1u};
+#pragma unroll
+    for (int t = 0; t < NTP; t++) {
+      if ((t & 3) == 0 && (uint32_t)t < nT) s = md5rand(s);
+      st[t * 64 + lane] = ((uint32_t)t < nT) ? sample_value<T>(u4_get(s, t & 3), bnd.tlo, bnd.thi)
+                                             : pad_threshold<T>();
+    }
+  }
+  __syncthreads();
+  // ranks L_t = #{t' < nT : thr_t' < thr_t}, one (t, f) pair per thread step
+  for (uint32_t t = tid >> 6; t < (uint32_t)NTP; t += HIST_WAVES) {
+    T th = st[t * 64 + lane];
+    uint32_t L = 0;
+#pragma unroll
+    for (int t2 = 0; t2 < NTP; t2++) L += ((uint32_t)t2 < nT && st[t2 * 64 + lane] < th) ? 1u : 0u;
+    sl[t * 64 + lane] = (uint8_t)L;
+  }
+  T p[FD];
+#pragma unroll
+  for (int j = 0; j < FD; j++) p[j] = sp[j * 64 + lane];
+  T thr[NTP];
+#pragma unroll
+  for (int t = 0; t < NTP; t++) thr[t] = st[t * 64 + lane];
+
+  // Sample loop.  r(v) = #{t : !(v <= thr_t)} is #{t : thr_t < v} for a
+  // finite v and nT for NaN (padded slots never count for finite v).
+  // v <= thr_t  <=>  r(v) <= L_t:  if v <= thr_t every thr below v is below
+  // thr_t; if v > thr_t then thr_t and every thr <= thr_t are below v, so
+  // r(v) >= L_t + 1.
+  const uint32_t end = it.begin + it.count;
+  for (uint32_t i = it.begin + wave; i < end; i += HIST_WAVES) {
+    const uint2 r = recs[i];
+    const ImgDesc d = imgs[rec_img(r)];
+    ImageView<typename F::ImgType, F::nChannels> view{pixels + d.off, (int)d.w, (int)d.h};
+    const T v = F::eval(view, rec_x(r), rec_y(r), p);
+    uint32_t rk = 0;
+#pragma unroll
+    for (int t = 0; t < NTP; t++) rk += (v <= thr[t]) ? 0u : 1u;
+    rk = rk < nT ? rk : nT;
+    if (fv) atomicAdd(&bins[(rec_cls(r) * RW + (rk >> 1)) * 64u + lane], 1u << ((rk & 1u) * 16u));
+  }
+  __syncthreads();
+
+  // Flush: per class (classes strided over waves) turn this lane's rank bins
+  // into prefix counts in place, then count(c, t) = prefix[L_t].
+  uint8_t L[NTP];
+#pragma unroll
+  for (int t = 0; t < NTP; t++) L[t] = sl[t * 64 + lane];
+  const bool atom = slot_atomic[it.slot] != 0;
+  const size_t per_class = (size_t)nT * nF;
+  uint32_t *Hs = H + (size_t)it.slot * nC * per_class;
+  for (uint32_t c = wave; c < nC; c += HIST_WAVES) {
+    uint32_t *b = bins + (size_t)c * RW * 64u + lane;
+    uint32_t run = 0;
+#pragma unroll
+    for (int j = 0; j < RW; j++) {
+      uint32_t w = b[j * 64];
+      uint32_t lo = run + (w & 0xffffu);
+      uint32_t hi = lo + (w >> 16);
+      run = hi;
+      b[j * 64] = lo | (hi << 16);
+    }
+    if (!fv) continue;
+    uint32_t *out = Hs + (size_t)c * per_class + f;
+#pragma unroll
+    for (int t = 0; t < NTP; t++) {
+      if ((uint32_t)t >= nT) break;
+      const uint32_t l = L[t];
+      const uint32_t val = (b[(l >> 1) * 64u] >> ((l & 1u) * 16u)) & 0xffffu;
+      if (atom) {
+        if (val) atomicAdd(out + (size_t)t * nF, val);
+      } else {
+        out[(size_t)t * nF] = val;
+      }
+    }
+  }
+}
+
+// Zero the H region of every slot that several chunks add into.
+__global__ void k_zero_slots(uint32_t *__restrict__ H, const uint32_t *__restrict__ slots, uint32_t n_slots,
+                             size_t per_slot)
+{
+  const size_t total = (size_t)n_slots * per_slot;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+    const size_t s = i / per_slot, o = i % per_slot;
+    H[(size_t)slots[s] * per_slot + o] = 0u;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// (b) split search.  Every thread scans pairs k = t*nF + f in increasing
+// order keeping the first maximum (strict >), then (gain, k) keys are
+// max-reduced over the wave, the block and (atomically) over the blocks of
+// the slot: the larger key is the larger gain, then the smaller k — the
+// reference's first maximum (learn_best_feature.cl:96-101 together with
+// std::max_element, cl_tree_trainer_impl_learn_best_featthr.hpp:111-112).
+// ---------------------------------------------------------------------------
+constexpr int SPLIT_THREADS = 256;
+
+template <int NCM>
+__device__ __forceinline__ float node_entropy(const uint32_t *tot, uint32_t nC, int &N)
+{
+  int n = 0;
+#pragma unroll
+  for (int l = 0; l < NCM; l++)
+    if ((uint32_t)l < nC) n += (int)tot[l];
+  N = n;
+  const float fN = (float)n;
+  float h = 0.0f;
+#pragma unroll
+  for (int l = 0; l < NCM; l++)
+    if ((uint32_t)l < nC) h -= plogp((float)tot[l] / fN);
+  return h;
+}
+
+// Gain of one pair given left counts; op order of learn_best_feature.cl:63-93
+// (h is the node entropy, the same value the kernel recomputes per pair).
+template <int NCM>
+__device__ __forceinline__ float pair_gain(const uint32_t *left, const uint32_t *tot, uint32_t nC, float h, int N)
+{
+  int nL = 0;
+#pragma unroll
+  for (int l = 0; l < NCM; l++)
+    if ((uint32_t)l < nC) nL += (int)left[l];
+  const int nR = N - nL;
+  const float fN = (float)N, fL = (float)nL, fR = (float)nR;
+  float hL = 0.0f, hR = 0.0f;
+#pragma unroll
+  for (int l = 0; l < NCM; l++) {
+    if ((uint32_t)l < nC) {
+      float tmp = nL ? (float)left[l] / fL : 0.0f;
+      hL -= plogp(tmp);
+      tmp = nR ? (float)(tot[l] - left[l]) / fR : 0.0f;
+      hR -= plogp(tmp);
+    }
+  }
+  const float a = fL / fN * hL;
+  const float b = fR / fN * hR;
+  return h - (a + b);
+}
+
+__device__ __forceinline__ uint64_t wave_max_u64(uint64_t v)
+{
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    uint64_t w = __shfl_xor(v, o, 64);
+    v = w > v ? w : v;
+  }
+  return v;
+}
+
+template <int NCM>
+__global__ __launch_bounds__(SPLIT_THREADS) void k_split(const uint32_t *__restrict__ H,
+                                                         const uint32_t *__restrict__ tot, uint32_t nC,
+                                                         uint32_t nT, uint32_t nF, uint32_t blocks_per_slot,
+                                                         uint32_t pairs_per_block,
+                                                         unsigned long long *__restrict__ best,
+                                                         float *__restrict__ all_gains)
+{
+  __shared__ uint64_t red[SPLIT_THREADS / 64];
+  const uint32_t slot = blockIdx.x / blocks_per_slot;
+  const uint32_t part = blockIdx.x % blocks_per_slot;
+  const uint32_t npairs = nT * nF;
+  uint32_t tv[NCM];
+#pragma unroll
+  for (int l = 0; l < NCM; l++) tv[l] = ((uint32_t)l < nC) ? tot[(size_t)slot * nC + l] : 0u;
+  int N;
+  const float h = node_entropy<NCM>(tv, nC, N);
+  const uint32_t *Hs = H + (size_t)slot * nC * npairs;
+  const uint32_t k0 = part * pairs_per_block;
+  const uint32_t k1 = min(npairs, k0 + pairs_per_block);
+  uint64_t key = 0;
+  for (uint32_t k = k0 + threadIdx.x; k < k1; k += SPLIT_THREADS) {
+    uint32_t left[NCM];
+#pragma unroll
+    for (int l = 0; l < NCM; l++) left[l] = ((uint32_t)l < nC) ? Hs[(size_t)l * npairs + k] : 0u;
+    const float g = pair_gain<NCM>(left, tv, nC, h, N);
+    if (all_gains) all_gains[(size_t)slot * npairs + k] = g;
+    const uint64_t kk = split_key(g, k);
+    key = kk > key ? kk : key;
+  }
+  key = wave_max_u64(key);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = key;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t m = red[0];
+#pragma unroll
+    for (int w = 1; w < SPLIT_THREADS / 64; w++) m = red[w] > m ? red[w] : m;
+    atomicMax(best + slot, (unsigned long long)m);
+  }
+}
+
+// Winner of each slot: res[slot] = {k, gain bits, left[0..nC)}.
+__global__ void k_finalize(const uint32_t *__restrict__ H, const unsigned long long *__restrict__ best,
+                           uint32_t n_slots, uint32_t nC, uint32_t nT, uint32_t nF, uint32_t *__restrict__ res)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t slot = i / (nC + 2), c = i % (nC + 2);
+  if (slot >= n_slots) return;
+  const uint64_t key = best[slot];
+  uint32_t k = key_index(key);
+  if (k >= nT * nF) k = 0;   // unreachable: every slot evaluates pair 0
+  uint32_t *r = res + (size_t)slot * (nC + 2);
+  if (c == 0) {
+    r[0] = k;
+  } else if (c == 1) {
+    const float g = key_gain(key);
+    r[1] = __float_as_uint(g);
+  } else {
+    r[c] = H[((size_t)slot * nC + (c - 2)) * nT * nF + k];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Routing of one level (replaces re-running predict.cl from the root at
+// every level, cl_tree_trainer_impl_traverse_ts.hpp:211-244): each active
+// sample of a split node moves to child leftChild + (v <= thr ? 0 : 1) and is
+// appended to that child's run of the next active list if the child is in
+// the next frontier.  Appends are wave-aggregated (one atomic per distinct
+// destination per wave).  child_count[2*slot+side] counts every routed
+// sample so the host can check it against the split's class counts.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t aggregated_append(uint32_t *ctr, int key, bool active)
+{
+  // returns this lane's position; lanes with equal key share one atomic
+  uint64_t pending = __ballot(active);
+  uint32_t pos = 0;
+  const uint32_t lane = threadIdx.x & 63u;
+  while (pending) {
+    const int leader = __ffsll((unsigned long long)pending) - 1;
+    const int lkey = __shfl(key, leader, 64);
+    const uint64_t grp = __ballot(active && key == lkey) & pending;
+    uint32_t base = 0;
+    if ((int)lane == leader) base = atomicAdd(ctr + lkey, (uint32_t)__popcll(grp));
+    base = __shfl(base, leader, 64);
+    if ((grp >> lane) & 1ull) pos = base + (uint32_t)__popcll(grp & ((1ull << lane) - 1ull));
+    pending &= ~grp;
+  }
+  return pos;
+}
+
+template <class F>
+__global__ __launch_bounds__(256) void k_route(const typename F::ImgType *__restrict__ pixels,
+                                               const ImgDesc *__restrict__ imgs, const uint2 *__restrict__ recs,
+                                               const uint32_t *__restrict__ rec_slot, uint32_t n,
+                                               const int32_t *__restrict__ slot_node,
+                                               const int32_t *__restrict__ tree_left,
+                                               const typename F::FeatType *__restrict__ tree_feat,
+                                               const typename F::FeatType *__restrict__ tree_thr,
+                                               const int32_t *__restrict__ slot_next, uint32_t *__restrict__ cursor,
+                                               uint32_t *__restrict__ child_count, uint2 *__restrict__ out_recs,
+                                               uint32_t *__restrict__ out_slot, uint32_t out_cap)
+{
+  using T = typename F::FeatType;
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  bool live = i < n;
+  uint32_t slot = 0, side = 0;
+  uint2 r = make_uint2(0u, 0u);
+  if (live) {
+    slot = rec_slot[i];
+    const int32_t node = slot_node[slot];
+    const int32_t lc = tree_left[node];
+    if (lc < 0) {
+      live = false;
+    } else {
+      r = recs[i];
+      const ImgDesc d = imgs[rec_img(r)];
+      ImageView<typename F::ImgType, F::nChannels> view{pixels + d.off, (int)d.w, (int)d.h};
+      T p[F::FeatDim];
+#pragma unroll
+      for (int j = 0; j < F::FeatDim; j++) p[j] = tree_feat[(size_t)node * F::FeatDim + j];
+      const T v = F::eval(view, rec_x(r), rec_y(r), p);
+      side = (v <= tree_thr[node]) ? 0u : 1u;
+    }
+  }
+  const int ckey = (int)(2u * slot + side);
+  (void)aggregated_append(child_count, ckey, live);
+  const int ns = live ? slot_next[ckey] : -1;
+  const bool go = live && ns >= 0;
+  const uint32_t pos = aggregated_append(cursor, ns < 0 ? 0 : ns, go);
+  // out_cap bounds the write even if routing disagreed with the split
+  // counts; the host compares child_count with them and reports it.
+  if (go && pos < out_cap) {
+    out_recs[pos] = r;
+    out_slot[pos] = (uint32_t)ns;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// (c) forest prediction.  Node records are packed breadth-first per tree
+// (one aligned load per visited node).  A block covers 256 consecutive
+// pixels of one frame; blocks map to frames through a prefix table.
+// Posteriors are summed in tree order in FP32 and divided by nTrees
+// (cl_classifier_impl.hpp:533-552); output is planar [c][y][x] per frame.
+// Masked-out pixels get posterior 0 and leaf id -1.
+// ---------------------------------------------------------------------------
+template <typename T, int FD>
+struct alignas(FD * sizeof(T) + 8 <= 16 ? 16 : 32) NodeRec {
+  int32_t left;
+  T thr;
+  T p[FD];
+};
+
+struct FrameDesc {
+  uint64_t img_off;    // element offset of the frame's pixels
+  uint64_t post_off;   // element offset of the frame's posterior [nC][h][w]
+  uint64_t leaf_off;   // element offset of the frame's leaf ids [tree][h][w]
+  uint64_t mask_off;   // byte offset of the frame's mask (if masks given)
+  uint32_t w, h;
+  uint32_t block0;     // first block of the frame
+  uint32_t pad;
+};
+
+template <class F, int NCM>
+__global__ __launch_bounds__(256) void k_predict(
+    const typename F::ImgType *__restrict__ pixels, const FrameDesc *__restrict__ frames, uint32_t n_frames,
+    const uint8_t *__restrict__ masks, const NodeRec<typename F::FeatType, F::FeatDim> *__restrict__ nodes,
+    const uint64_t *__restrict__ tree_off, uint32_t n_trees, uint32_t max_steps,
+    const float *__restrict__ posts, const uint64_t *__restrict__ post_off, uint32_t nC,
+    float *__restrict__ out_post, int32_t *__restrict__ out_leaf)
+{
+  using T = typename F::FeatType;
+  // frame of this block: last frame with block0 <= blockIdx.x (uniform)
+  uint32_t lo = 0, hi = n_frames;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (frames[mid].block0 <= blockIdx.x) lo = mid; else hi = mid;
+  }
+  const FrameDesc fd = frames[lo];
+  const uint32_t npx = fd.w * fd.h;
+  const uint32_t px = (blockIdx.x - fd.block0) * 256u + threadIdx.x;
+  if (px >= npx) return;
+  const int x = (int)(px % fd.w), y = (int)(px / fd.w);
+  const bool on = masks ? masks[fd.mask_off + px] != 0 : true;
+  ImageView<typename F::ImgType, F::nChannels> view{pixels + fd.img_off, (int)fd.w, (int)fd.h};
+  float acc[NCM];
+#pragma unroll
+  for (int c = 0; c < NCM; c++) acc[c] = 0.0f;
+  for (uint32_t t = 0; t < n_trees; t++) {
+    int32_t n = -1;
+    if (on) {
+      const NodeRec<T, F::FeatDim> *tr = nodes + tree_off[t];
+      n = 0;
+      for (uint32_t s = 0; s < max_steps; s++) {
+        const NodeRec<T, F::FeatDim> rec = tr[n];
+        if (rec.left < 0) break;
+        const T v = F::eval(view, x, y, rec.p);
+        n = rec.left + ((v <= rec.thr) ? 0 : 1);
+      }
+      if (out_post) {
+        const float *pp = posts + post_off[t] + (size_t)n * nC;
+#pragma unroll
+        for (int c = 0; c < NCM; c++)
+          if ((uint32_t)c < nC) acc[c] += pp[c];
+      }
+    }
+    if (out_leaf) out_leaf[fd.leaf_off + (size_t)t * npx + px] = n;
+  }
+  if (out_post) {
+    const float nt = (float)n_trees;
+#pragma unroll
+    for (int c = 0; c < NCM; c++)
+      if ((uint32_t)c < nC) out_post[fd.post_off + (size_t)c * npx + px] = on ? acc[c] / nt : 0.0f;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Small unit kernels used by the parity tests of the individual steps.
+// ---------------------------------------------------------------------------
+template <class F>
+__global__ void k_candidates(uint32_t tree, const int32_t *__restrict__ nodes, uint32_t n_nodes, uint32_t nF,
+                             uint32_t nT, Bounds<F> bnd, typename F::FeatType *__restrict__ params,
+                             typename F::FeatType *__restrict__ thrs)
+{
+  using T = typename F::FeatType;
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_nodes * nF) return;
+  const uint32_t nd = (uint32_t)nodes[i / nF], f = i % nF;
+  T p[F::FeatDim];
+  candidate_params<T, F::FeatDim>(tree, nd, f, bnd.lo, bnd.hi, p);
+  for (int j = 0; j < F::FeatDim; j++) params[(size_t)i * F::FeatDim + j] = p[j];
+  U4 s = {tree, nd, f, 1u};
+  for (uint32_t t = 0; t < nT; t++) {
+    if ((t & 3) == 0) s = md5rand(s);
+    thrs[(size_t)i * nT + t] = sample_value<T>(u4_get(s, t & 3), bnd.tlo, bnd.thi);
+  }
+}
+
+template <class F>
+__global__ void k_feature_eval(const typename F::ImgType *__restrict__ pixels, uint32_t w, uint32_t h,
+                               const int32_t *__restrict__ xy, const typename F::FeatType *__restrict__ params,
+                               uint32_t n, typename F::FeatType *__restrict__ out)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  ImageView<typename F::ImgType, F::nChannels> view{pixels, (int)w, (int)h};
+  out[i] = F::eval(view, xy[2 * i], xy[2 * i + 1], params + (size_t)i * F::FeatDim);
+}
+
+__global__ void k_log2_det(const float *__restrict__ x, float *__restrict__ y, uint32_t n)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = log2_det(x[i]);
+}
+
+__global__ void k_md5(const uint4 *__restrict__ in, uint4 *__restrict__ out, uint32_t n)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint4 a = in[i];
+  const U4 r = md5rand(U4{a.x, a.y, a.z, a.w});
+  out[i] = make_uint4(r.x, r.y, r.z, r.w);
+}
+
+}  // namespace pd

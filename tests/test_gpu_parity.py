@@ -1,0 +1,369 @@
+"""GPU parity tests: every step of the hot path through the C ABI against the
+CPU oracle on the same seeded inputs.  Integer / index results and FP32
+values are compared bit for bit; FP64 gains within 1e-12; posteriors within
+1e-6 relative of an FP64 average."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import padenti_amd as pa
+from padenti_amd import _lib as L
+from padenti_amd import synth
+from oracle import oracle as orc
+from tests import helpers as hp
+
+pytestmark = pytest.mark.gpu
+
+KINDS = [pa.PD_FEAT_DEPTH_F32, pa.PD_FEAT_DEPTH_U16, pa.PD_FEAT_RGBD_U16X4, pa.PD_FEAT_DEMO_S16]
+
+
+def test_md5rand_device(gpu):
+    rng = np.random.default_rng(1)
+    seeds = rng.integers(0, 2 ** 32, size=(4096, 4), dtype=np.uint64).astype(np.uint32)
+    seeds[0] = 0
+    seeds[1] = 0xffffffff
+    out = np.zeros_like(seeds)
+    L.check(L.lib().pd_md5rand(0, len(seeds), L.ptr(seeds), L.ptr(out)))
+    for i in range(0, len(seeds), 97):
+        assert out[i].tolist() == hp.md5rand(seeds[i])
+    want = np.stack([orc.md5rand(s) for s in seeds])
+    assert np.array_equal(out, want)
+
+
+def test_log2_det_device_bit_exact(gpu):
+    rng = np.random.default_rng(2)
+    x = np.concatenate([rng.uniform(1e-6, 1.0, 100000), np.float32(1.0) / np.arange(1, 5000)]).astype(np.float32)
+    y = np.zeros_like(x)
+    L.check(L.lib().pd_log2_det(0, len(x), L.ptr(x), L.ptr(y)))
+    want = np.array([orc.log2_det(v) for v in x], dtype=np.float32)
+    assert y.tobytes() == want.tobytes()
+
+
+def _rand_image(kind, rng, w, h):
+    if kind == pa.PD_FEAT_DEPTH_F32:
+        im = rng.uniform(0.5, 6.0, (h, w)).astype(np.float32)
+        im[rng.integers(0, h, 5), rng.integers(0, w, 5)] = 0.0
+        return im
+    if kind == pa.PD_FEAT_RGBD_U16X4:
+        im = rng.integers(0, 256, (h, w, 4)).astype(np.uint16)
+        im[..., 3] = rng.integers(400, 5000, (h, w))
+        im[rng.integers(0, h, 5), rng.integers(0, w, 5), 3] = 0
+        return im
+    im = rng.integers(400, 5000, (h, w)).astype(np.uint16)
+    im[rng.integers(0, h, 5), rng.integers(0, w, 5)] = 0
+    return im
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_feature_eval_device(gpu, kind):
+    rng = np.random.default_rng(3 + kind)
+    w, h = 61, 47
+    img = _rand_image(kind, rng, w, h)
+    n = 20000
+    xy = np.stack([rng.integers(0, w, n), rng.integers(0, h, n)], axis=1).astype(np.int32)
+    fd = L.FEATURES[kind][3]
+    scale = {0: 60.0, 1: 100000.0, 2: 100000.0, 3: 60.0}[kind]
+    prm = rng.uniform(-scale, scale, (n, fd))
+    if kind == pa.PD_FEAT_RGBD_U16X4:
+        prm[:, 4:] = rng.uniform(-0.5, 4.5, (n, 2))
+    prm = prm.astype(L.FEATURES[kind][2])
+    prm[:50] = np.round(prm[:50])       # exact .5 quotients occur with integral offsets
+    out = np.zeros(n, dtype=L.FEATURES[kind][2])
+    L.check(L.lib().pd_feature_eval(0, kind, L.ptr(img), w, h, n, L.ptr(xy), L.ptr(prm), L.ptr(out)))
+    want = np.array([orc.feature_at(kind, img, int(xy[i, 0]), int(xy[i, 1]), prm[i]) for i in range(n)])
+    assert np.array_equal(out.astype(np.float64), want)
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_candidate_tables_device(gpu, kind):
+    fd = L.FEATURES[kind][3]
+    lo = [-60.0, -60.0, -100.0, -100.0, 0.0, 0.0][:fd]
+    hi = [60.0, 60.0, 100.0, 100.0, 4.0, 4.0][:fd]
+    tb = (-200.0, 200.0) if kind == pa.PD_FEAT_DEMO_S16 else (-0.5, 0.5)
+    p = pa.TreeTrainerParameters(37, 13, lo, hi, tb[0], tb[1], 0)
+    nodes = np.array([0, 1, 2, 77, 65534, 524287], dtype=np.int32)
+    fdt = L.FEATURES[kind][2]
+    P = np.zeros((len(nodes), 37, fd), dtype=fdt)
+    T = np.zeros((len(nodes), 37, 13), dtype=fdt)
+    L.check(L.lib().pd_candidate_tables(0, kind, p.c(), 5, len(nodes), L.ptr(nodes), L.ptr(P), L.ptr(T)))
+    for i, n in enumerate(nodes):
+        for f in range(37):
+            assert P[i, f].tobytes() == orc.feature_params(kind, 5, int(n), f, lo, hi).tobytes()
+            assert T[i, f].tobytes() == orc.thresholds(kind, 5, int(n), f, 13, tb[0], tb[1]).tobytes()
+
+
+def _tiny_ts(kind, rng, n_img=3, w=40, h=30, nC=5, ns=300):
+    ts = pa.TrainingSet(nC)
+    for _ in range(n_img):
+        px = _rand_image(kind, rng, w, h)
+        lab = rng.integers(1, nC + 1, (h, w)).astype(np.uint8)
+        smp = np.sort(rng.integers(0, w * h, ns)).astype(np.uint32)
+        ts << pa.TrainingSetImage(px, w, h, lab, nC, smp)
+    return ts
+
+
+@pytest.mark.parametrize("kind,nF,nT,chunk", [
+    (pa.PD_FEAT_DEPTH_F32, 100, 10, 0),      # nF not a multiple of 64
+    (pa.PD_FEAT_DEPTH_U16, 130, 20, 64),     # several chunks per node -> atomic flush
+    (pa.PD_FEAT_DEPTH_U16, 64, 32, 0),
+    (pa.PD_FEAT_RGBD_U16X4, 70, 7, 100),
+    (pa.PD_FEAT_DEMO_S16, 40, 33, 0),        # 33 thresholds -> 48-slot rank bins
+    (pa.PD_FEAT_DEPTH_F32, 5, 1, 0),
+    (pa.PD_FEAT_DEPTH_F32, 3, 64, 0),
+])
+def test_hist_nodes_device(gpu, kind, nF, nT, chunk):
+    rng = np.random.default_rng(100 + nF + nT)
+    ts = _tiny_ts(kind, rng)
+    fd = L.FEATURES[kind][3]
+    if kind == pa.PD_FEAT_DEMO_S16:
+        lo, hi, tb = [-60] * fd, [60] * fd, (-300, 300)
+    elif kind == pa.PD_FEAT_DEPTH_F32:
+        lo, hi, tb = [-60.0] * fd, [60.0] * fd, (-0.8, 0.8)
+    else:
+        lo, hi, tb = [-100000.0] * fd, [100000.0] * fd, (-800.0, 800.0)
+        if kind == pa.PD_FEAT_RGBD_U16X4:
+            lo[4:], hi[4:] = [0.0, 0.0], [4.0, 4.0]
+    p = pa.TreeTrainerParameters(nF, nT, lo, hi, tb[0], tb[1], 0)
+    tr = pa.HIPTreeTrainer(kind, ts.n_classes)
+    tr.set_training_set(ts)
+    if chunk:
+        tr.set_chunk(chunk)
+    S = sum(im.samples.size for im in ts.images)
+    nodes = np.array([4, 0, 11, 12], dtype=np.int32)
+    sample_node = rng.choice([0, 4, 11, 12, -1, 3], S).astype(np.int32)
+    sample_node[:S // 3] = 0     # one big node
+    H = tr.hist_nodes(p, 2, nodes, sample_node)
+    want = orc.hist_nodes(orc.Tset(kind, ts), p, 2, nodes, sample_node)
+    assert np.array_equal(H, want)
+    tr.close()
+
+
+def test_split_search_device(gpu):
+    rng = np.random.default_rng(7)
+    for nC, nT, nF in [(4, 10, 100), (8, 20, 2000), (16, 32, 70), (21, 5, 9), (3, 2, 3)]:
+        nodes = 3
+        tot = rng.integers(1, 400, (nodes, nC)).astype(np.uint32)
+        H = np.stack([np.stack([rng.integers(0, t + 1, (nT, nF)) for t in tot[q]]) for q in range(nodes)])
+        H = H.astype(np.uint32)
+        # plant exact ties: the best pure split at two places
+        H[0, :, nT - 1, 0] = 0
+        H[0, 0, nT - 1, 0] = tot[0, 0]
+        H[0, :, 0, nF - 1] = H[0, :, nT - 1, 0]
+        bk = np.zeros(nodes, dtype=np.uint32)
+        bg = np.zeros(nodes, dtype=np.float32)
+        allg = np.zeros((nodes, nT * nF), dtype=np.float32)
+        L.check(L.lib().pd_split_search(0, nodes, nC, nT, nF, L.ptr(H), L.ptr(tot), L.ptr(bk), L.ptr(bg),
+                                        L.ptr(allg)))
+        for q in range(nodes):
+            k, g = orc.best_split(H[q], tot[q])
+            assert bk[q] == k and bg[q].tobytes() == np.float32(g).tobytes(), (nC, nT, nF, q)
+            for kk in range(0, nT * nF, max(1, nT * nF // 50)):
+                t, f = divmod(kk, nF)
+                assert allg[q, kk].tobytes() == orc.gain_f32(H[q, :, t, f], tot[q]).tobytes()
+        assert bk[0] == nF - 1    # (t=0, f=nF-1) precedes (t=nT-1, f=0) in threshold-major order
+
+
+def _check_tree_equal(tree, out, diag):
+    assert np.array_equal(tree.left_children, out["left_children"])
+    assert tree.features.tobytes() == out["features"].tobytes()
+    assert tree.thresholds.tobytes() == out["thresholds"].tobytes()
+    assert tree.posteriors.tobytes() == out["posteriors"].tobytes()
+    split = out["left_children"] >= 0
+    assert np.array_equal(diag["node_k"][split], out["node_k"][split])
+    assert diag["node_gain"][split].tobytes() == out["node_gain"][split].tobytes()
+    assert np.allclose(diag["node_gain64"][split], out["node_gain64"][split], rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("mode", [pa.PD_MODE_LEVEL_BATCHED, pa.PD_MODE_PER_NODE])
+def test_train_c1_bit_exact(gpu, mode):
+    """Config C1 in full: 32 images 160x120, 100 x 10, depth 8."""
+    ts = synth.training_set("c1")
+    p = synth.params_for("c1")
+    c = synth.CONFIGS["c1"]
+    tree = pa.Tree(0, c["depth"], c["kind"], c["n_classes"])
+    tr = pa.HIPTreeTrainer(c["kind"], c["n_classes"])
+    diag = tr.train(tree, ts, p, 1, c["depth"], mode=mode, diagnostics=True)
+    out = orc.train_tree(orc.Tset(c["kind"], ts), p, ts.priors, 0, c["depth"])
+    _check_tree_equal(tree, out, diag)
+    assert (tree.left_children >= 0).sum() > 20
+    st = tr.last_level_stats
+    assert st[0]["active_samples"] == 32 * 256 and st[0]["evals"] == 32 * 256 * 100 * 10
+    tr.close()
+
+
+def test_train_c2_shape_bit_exact(gpu):
+    """C2 shapes (320x240 uint16 capsules, 8 classes, 2000 x 20) on 12 images,
+    depth 10; a small histogram budget forces several slices per level."""
+    ts = synth.training_set("c2", n_images=12, samples=400)
+    p = synth.params_for("c2")
+    tree = pa.Tree(1, 10, pa.PD_FEAT_DEPTH_U16, 8)
+    tr = pa.HIPTreeTrainer(pa.PD_FEAT_DEPTH_U16, 8)
+    tr.set_training_set(ts)
+    tr.set_hist_budget(3 * 8 * 20 * 2000 * 4)
+    diag = tr.train(tree, ts, p, 1, 10, diagnostics=True)
+    assert max(s["slices"] for s in tr.last_level_stats) > 1
+    out = orc.train_tree(orc.Tset(pa.PD_FEAT_DEPTH_U16, ts), p, ts.priors, 1, 10)
+    _check_tree_equal(tree, out, diag)
+    tr.close()
+
+
+@pytest.mark.parametrize("kind", [pa.PD_FEAT_DEMO_S16, pa.PD_FEAT_RGBD_U16X4])
+def test_train_other_hooks_bit_exact(gpu, kind):
+    if kind == pa.PD_FEAT_DEMO_S16:
+        ts = synth.training_set("c2", n_images=6, samples=300)
+        p = pa.TreeTrainerParameters(128, 10, [-60, -60], [60, 60], -200, 200, 20)
+        nC = 8
+    else:
+        ts = synth.training_set("c4", n_images=6, samples=300, w=160, h=120)
+        p = synth.params_for("c4", n_features=150, n_thresholds=12)
+        nC = 21
+    tree = pa.Tree(3, 9, kind, nC)
+    tr = pa.HIPTreeTrainer(kind, nC)
+    diag = tr.train(tree, ts, p, 1, 9, diagnostics=True)
+    out = orc.train_tree(orc.Tset(kind, ts), p, ts.priors, 3, 9)
+    _check_tree_equal(tree, out, diag)
+    tr.close()
+
+
+def test_train_end_depth_and_root_gate(gpu):
+    ts = synth.training_set("c1", n_images=4, samples=64)
+    p = synth.params_for("c1", n_features=20, n_thresholds=4)
+    tree = pa.Tree(0, 8, pa.PD_FEAT_DEPTH_F32, 4)
+    tr = pa.HIPTreeTrainer(pa.PD_FEAT_DEPTH_F32, 4)
+    tr.train(tree, ts, p, 1, 4)            # train only 3 levels of a depth-8 tree
+    out = orc.train_tree(orc.Tset(0, ts), p, ts.priors, 0, 8, end_depth=4)
+    assert np.array_equal(tree.left_children, out["left_children"])
+    assert (tree.left_children[7:] == -2).all()
+    p.per_leaf_samples_thr = 10000         # root holds fewer samples: empty histogram, no split
+    tr.train(tree, ts, p, 1, 8)
+    assert (tree.left_children == -2).all()
+    with pytest.raises(ValueError):
+        tr.train(tree, ts, p, 2, 8)
+    tr.close()
+
+
+def _c1_forest(n_trees=3, depth=8):
+    ts = synth.training_set("c1")
+    p = synth.params_for("c1")
+    tr = pa.HIPTreeTrainer(pa.PD_FEAT_DEPTH_F32, 4)
+    trees = []
+    for t in range(n_trees):
+        tree = pa.Tree(t, depth, pa.PD_FEAT_DEPTH_F32, 4)
+        tr.train(tree, ts, p, 1, depth)
+        trees.append(tree)
+    tr.close()
+    return trees
+
+
+def test_predict_leaves_and_posterior(gpu):
+    trees = _c1_forest()
+    clf = pa.HIPClassifier(pa.PD_FEAT_DEPTH_F32, 4)
+    for t in trees:
+        clf << t
+    rng = np.random.default_rng(9)
+    for img, lab in synth.heldout("c1", n=4):
+        h, w = img.shape
+        mask = (rng.uniform(size=(h, w)) > 0.3).astype(np.uint8)
+        for m in (None, mask):
+            for ti, t in enumerate(trees):
+                got = clf.predict_leaves(ti, img, w, h, m)
+                assert np.array_equal(got, orc.predict_leaves(0, img, m, t))
+            post = clf.predict_posterior(img, w, h, m)
+            want = orc.predict_posterior(0, img, m, trees, 4)
+            assert post.tobytes() == want.tobytes()
+            p64 = np.zeros(post.shape)
+            for t in trees:
+                lv = orc.predict_leaves(0, img, m, t)
+                p64 += t.posteriors[np.maximum(lv, 0)].transpose(2, 0, 1)
+            on = (m if m is not None else np.ones((h, w), np.uint8))[None] > 0
+            p64 = np.where(on, p64 / len(trees), 0.0)
+            assert np.allclose(post, p64, rtol=1e-6, atol=1e-7)
+    clf.close()
+
+
+def test_predict_frames_graph(gpu):
+    """Mixed-size frames in one launch, eager and replayed from a hipGraph."""
+    trees = _c1_forest(n_trees=2, depth=7)
+    clf = pa.HIPClassifier(pa.PD_FEAT_DEPTH_F32, 4)
+    for t in trees:
+        clf << t
+    sizes = [(160, 120), (64, 48), (200, 90), (33, 17)]
+    frames = [synth.gen_image(0, 0, 5000 + i, w, h, 4)[0] for i, (w, h) in enumerate(sizes)]
+    npx = [w * h for w, h in sizes]
+    img_off = np.concatenate([[0], np.cumsum(npx)[:-1]]).astype(np.uint64)
+    post_off = (img_off * 4).astype(np.uint64)
+    leaf_off = (img_off * 2).astype(np.uint64)
+    masks = [np.ones(n, np.uint8) for n in npx]
+    masks[1][::3] = 0
+    host_pix = np.concatenate([f.reshape(-1) for f in frames]).astype(np.float32)
+    host_mask = np.concatenate(masks)
+    lib = L.lib()
+
+    def dalloc(nbytes):
+        p = ctypes.c_void_p()
+        L.check(lib.pd_dev_alloc(0, nbytes, ctypes.byref(p)))
+        return p
+
+    d_pix, d_mask = dalloc(host_pix.nbytes), dalloc(host_mask.nbytes)
+    d_post, d_leaf = dalloc(sum(npx) * 4 * 4), dalloc(sum(npx) * 2 * 4)
+    L.check(lib.pd_memcpy_h2d(d_pix, L.ptr(host_pix), host_pix.nbytes))
+    L.check(lib.pd_memcpy_h2d(d_mask, L.ptr(host_mask), host_mask.nbytes))
+    w = np.array([s[0] for s in sizes], np.uint32)
+    h = np.array([s[1] for s in sizes], np.uint32)
+    mask_off = img_off.copy()
+    ms = ctypes.c_float()
+    outs = []
+    for use_graph in (0, 1, 1, 1):
+        L.check(lib.pd_forest_predict_frames_dev(clf._h, len(sizes), d_pix, L.ptr(img_off), L.ptr(w), L.ptr(h),
+                                                 d_mask, L.ptr(mask_off), d_post, L.ptr(post_off), d_leaf,
+                                                 L.ptr(leaf_off), use_graph, ctypes.byref(ms)))
+        post = np.zeros(sum(npx) * 4, np.float32)
+        leaf = np.zeros(sum(npx) * 2, np.int32)
+        L.check(lib.pd_memcpy_d2h(L.ptr(post), d_post, post.nbytes))
+        L.check(lib.pd_memcpy_d2h(L.ptr(leaf), d_leaf, leaf.nbytes))
+        outs.append((post, leaf))
+        # eager work between replays
+        clf.predict_posterior(frames[0], 160, 120)
+    for post, leaf in outs[1:]:
+        assert post.tobytes() == outs[0][0].tobytes() and np.array_equal(leaf, outs[0][1])
+    post, leaf = outs[0]
+    for i, (fw, fh) in enumerate(sizes):
+        m = masks[i].reshape(fh, fw)
+        want = orc.predict_posterior(0, frames[i], m, trees, 4).reshape(-1)
+        got = post[post_off[i]:post_off[i] + 4 * npx[i]]
+        assert got.tobytes() == want.tobytes(), i
+        for t in range(2):
+            wl = orc.predict_leaves(0, frames[i], m, trees[t]).reshape(-1)
+            gl = leaf[leaf_off[i] + t * npx[i]:leaf_off[i] + (t + 1) * npx[i]]
+            assert np.array_equal(gl, wl)
+    for p in (d_pix, d_mask, d_post, d_leaf):
+        lib.pd_dev_free(p)
+    clf.close()
+
+
+def test_error_paths(gpu):
+    tr = pa.HIPTreeTrainer(pa.PD_FEAT_DEPTH_F32, 4)
+    tree = pa.Tree(0, 5, pa.PD_FEAT_DEPTH_F32, 4)
+    ts = synth.training_set("c1", n_images=2, samples=16)
+    p = synth.params_for("c1", n_features=8, n_thresholds=65)
+    with pytest.raises(pa.PdError) as e:
+        tr.train(tree, ts, p, 1, 5)
+    assert e.value.code == -1 and "n_thresholds" in str(e.value)
+    bad = pa.TrainingSet(4)
+    img = np.ones((8, 8), np.float32)
+    lab = np.zeros((8, 8), np.uint8)
+    lab[0, 0] = 1
+    bad << pa.TrainingSetImage(img, 8, 8, lab, 4, np.array([0, 5], np.uint32))   # pixel 5 is unlabelled
+    with pytest.raises(pa.PdError) as e:
+        tr.set_training_set(bad)
+    assert e.value.code == -1 and "unlabelled" in str(e.value)
+    clf = pa.HIPClassifier(pa.PD_FEAT_DEPTH_F32, 4)
+    t = pa.Tree(0, 3, pa.PD_FEAT_DEPTH_F32, 4)
+    t.left_children[0] = 5    # not 2*0+1
+    with pytest.raises(pa.PdError):
+        clf << t
+    with pytest.raises(pa.PdError):
+        clf.predict_posterior(img, 8, 8)    # no trees
+    clf.close()
+    tr.close()
