@@ -3,12 +3,18 @@
 // height, coords, treeLeftChildren, treePosteriors, imageNodesID, features,
 // featDim)` (test/feature.cl:29-34; contract in src/padenti/padenti.hpp.in:
 // 170-175).  A functor is a struct with
-//   using ImgType, FeatType; static constexpr nChannels, FeatDim;
-//   static FeatType eval(const ImageView<ImgType,nChannels>&, int x, int y,
-//                        const FeatType* params);
-// compiled for host and device.  Reads outside the image follow the
-// reference sampler (CLK_ADDRESS_CLAMP + an explicit border test in the hook):
-// each functor returns its own background value there.
+//   using ImgType, FeatType, Prep; static constexpr nChannels, FeatDim;
+//   static Prep prepare(const ImageView<ImgType,nChannels>&, int x, int y);
+//   static FeatType eval_prep(const ImageView<ImgType,nChannels>&, int x, int y,
+//                             Prep, const FeatType* params);
+//   static FeatType eval(view, x, y, params)
+//       == eval_prep(view, x, y, prepare(view, x, y), params);
+// compiled for host and device.  `prepare` is the per-pixel part shared by
+// every candidate (here the centre depth); the trainer evaluates it once per
+// sample and the predictor once per pixel.  Prep must fit in 64 bits.
+// Reads outside the image follow the reference sampler (CLK_ADDRESS_CLAMP
+// plus an explicit border test in the hook): each functor returns its own
+// background value there.
 //
 // A response goes to the left child when response <= threshold
 // (hist_update.cl:154, predict.cl:75).
@@ -38,28 +44,97 @@ struct ImageView {
 // which keeps the float->int conversion defined on every target.
 PD_HD bool offset_ok(float q) { return fabsf(q) < 1048576.0f; }
 
+// Denominator prepared once per pixel: d and a refined reciprocal y.
+struct DivPrep {
+  float d, y;
+};
+
+PD_HD DivPrep div_prep(float d)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+  const float y0 = __builtin_amdgcn_rcpf(d);
+  const float e = __builtin_fmaf(-d, y0, 1.0f);
+  return DivPrep{d, __builtin_fmaf(e, y0, y0)};
+#else
+  return DivPrep{d, 1.0f / d};
+#endif
+}
+
+// p / d, IEEE round-to-nearest.  On the device, for d in [2^-16, 2^16] this
+// is the compiler's own f32 division sequence for gfx950 (reciprocal, one
+// Newton step, quotient, two FMA residual corrections) with its operand
+// scaling and special-case fix-up left out, both identities there: the
+// quotient bits equal `p / d` for |p| in [2^-60, 2^60], and outside that
+// range the rounded offsets computed from it are the same (0, or beyond
+// 2^20 / non-finite).  The reciprocal part is shared by all candidates of a
+// pixel.  Any other d takes the plain division.
+PD_HD float div_by(float p, const DivPrep &dp, bool fast)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+  if (fast) {
+    float q = p * dp.y;
+    float r = __builtin_fmaf(-dp.d, q, p);
+    q = __builtin_fmaf(r, dp.y, q);
+    r = __builtin_fmaf(-dp.d, q, p);
+    return __builtin_fmaf(r, dp.y, q);
+  }
+#endif
+  (void)fast;
+  return p / dp.d;
+}
+PD_HD bool div_fast_ok(const DivPrep &dp) { return dp.d >= 0x1p-16f && dp.d <= 0x1p16f; }
+
+// Integer offset round(q) (half away from zero) of a quotient, with every
+// quotient whose magnitude is not below 2^20 (or NaN) mapped to +-2^20, an
+// offset that leaves any image (width, height < 2^16): the same pixel/border
+// decision as roundf + offset_ok.  For 0.5 <= |q| <= 2^20, q + copysign(0.5, q)
+// rounds to a value with the same integer part as the exact sum (0.5 is a
+// multiple of ulp(q) there and a rounding never reaches the next integer),
+// so truncation gives round(q); |q| < 0.5 gives 0.
+PD_HD int offset_of(float q)
+{
+  q = fminf(fmaxf(q, -1048576.0f), 1048576.0f);
+  const int n = (int)(q + copysignf(0.5f, q));
+  return fabsf(q) < 0.5f ? 0 : n;
+}
+
+PD_HD int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// Read of channel `ch` at x + round(px/d), y + round(py/d), or `bg` when the
+// offset leaves the image.  Branch-free: the address is clamped into the
+// image and the background selected afterwards (same value as testing first).
+template <typename ImgT, int NCH>
+PD_HD float probe_div(const ImageView<ImgT, NCH> &im, int x, int y, const DivPrep &dp, bool fast, float px,
+                      float py, int ch, float bg)
+{
+  const int ox = x + offset_of(div_by(px, dp, fast));
+  const int oy = y + offset_of(div_by(py, dp, fast));
+  const bool in = (uint32_t)ox < (uint32_t)im.w && (uint32_t)oy < (uint32_t)im.h;
+  const float v = (float)im.at(clampi(ox, 0, im.w - 1), clampi(oy, 0, im.h - 1), ch);
+  return in ? v : bg;
+}
+
 // C1: float depth in metres.  f = I(x + round(u/d(x))) - I(x + round(v/d(x))),
 // p = (ux, uy, vx, vy) in pixel*m, out-of-image reads 10 m, d(x) <= 0 -> 0.
 struct DepthF32Feature {
   using ImgType = float;
   using FeatType = float;
+  using Prep = DivPrep;
   static constexpr int nChannels = 1;
   static constexpr int FeatDim = 4;
   static constexpr int kind = 0;
-  PD_HD static float probe(const ImageView<float, 1> &im, int x, int y, float d, float px, float py)
+  PD_HD static Prep prepare(const ImageView<float, 1> &im, int x, int y) { return div_prep(im.at(x, y, 0)); }
+  PD_HD static float eval_prep(const ImageView<float, 1> &im, int x, int y, Prep dp, const float *p)
   {
-    float qx = roundf(px / d), qy = roundf(py / d);
-    if (!offset_ok(qx) || !offset_ok(qy)) return 10.0f;
-    int ox = x + (int)qx, oy = y + (int)qy;
-    return im.inside(ox, oy) ? im.at(ox, oy, 0) : 10.0f;
+    if (!(dp.d > 0.0f)) return 0.0f;
+    const bool fast = div_fast_ok(dp);
+    float a = probe_div(im, x, y, dp, fast, p[0], p[1], 0, 10.0f);
+    float b = probe_div(im, x, y, dp, fast, p[2], p[3], 0, 10.0f);
+    return a - b;
   }
   PD_HD static float eval(const ImageView<float, 1> &im, int x, int y, const float *p)
   {
-    float d = im.at(x, y, 0);
-    if (!(d > 0.0f)) return 0.0f;
-    float a = probe(im, x, y, d, p[0], p[1]);
-    float b = probe(im, x, y, d, p[2], p[3]);
-    return a - b;
+    return eval_prep(im, x, y, prepare(im, x, y), p);
   }
 };
 
@@ -68,23 +143,25 @@ struct DepthF32Feature {
 struct DepthU16Feature {
   using ImgType = uint16_t;
   using FeatType = float;
+  using Prep = DivPrep;
   static constexpr int nChannels = 1;
   static constexpr int FeatDim = 4;
   static constexpr int kind = 1;
-  PD_HD static float probe(const ImageView<uint16_t, 1> &im, int x, int y, float d, float px, float py)
+  PD_HD static Prep prepare(const ImageView<uint16_t, 1> &im, int x, int y)
   {
-    float qx = roundf(px / d), qy = roundf(py / d);
-    if (!offset_ok(qx) || !offset_ok(qy)) return 10000.0f;
-    int ox = x + (int)qx, oy = y + (int)qy;
-    return im.inside(ox, oy) ? (float)im.at(ox, oy, 0) : 10000.0f;
+    return div_prep((float)im.at(x, y, 0));
+  }
+  PD_HD static float eval_prep(const ImageView<uint16_t, 1> &im, int x, int y, Prep dp, const float *p)
+  {
+    if (!(dp.d > 0.0f)) return 0.0f;
+    const bool fast = div_fast_ok(dp);
+    float a = probe_div(im, x, y, dp, fast, p[0], p[1], 0, 10000.0f);
+    float b = probe_div(im, x, y, dp, fast, p[2], p[3], 0, 10000.0f);
+    return a - b;
   }
   PD_HD static float eval(const ImageView<uint16_t, 1> &im, int x, int y, const float *p)
   {
-    float d = (float)im.at(x, y, 0);
-    if (!(d > 0.0f)) return 0.0f;
-    float a = probe(im, x, y, d, p[0], p[1]);
-    float b = probe(im, x, y, d, p[2], p[3]);
-    return a - b;
+    return eval_prep(im, x, y, prepare(im, x, y), p);
   }
 };
 
@@ -95,6 +172,7 @@ struct DepthU16Feature {
 struct RGBDFeature {
   using ImgType = uint16_t;
   using FeatType = float;
+  using Prep = DivPrep;
   static constexpr int nChannels = 4;
   static constexpr int FeatDim = 6;
   static constexpr int kind = 2;
@@ -104,21 +182,21 @@ struct RGBDFeature {
     if (c >= 3.0f) return 3;
     return (int)c;
   }
-  PD_HD static float probe(const ImageView<uint16_t, 4> &im, int x, int y, float d, float px, float py,
-                           int ch)
+  PD_HD static Prep prepare(const ImageView<uint16_t, 4> &im, int x, int y)
   {
-    float qx = roundf(px / d), qy = roundf(py / d);
-    if (!offset_ok(qx) || !offset_ok(qy)) return 10000.0f;
-    int ox = x + (int)qx, oy = y + (int)qy;
-    return im.inside(ox, oy) ? (float)im.at(ox, oy, ch) : 10000.0f;
+    return div_prep((float)im.at(x, y, 3));
+  }
+  PD_HD static float eval_prep(const ImageView<uint16_t, 4> &im, int x, int y, Prep dp, const float *p)
+  {
+    if (!(dp.d > 0.0f)) return 0.0f;
+    const bool fast = div_fast_ok(dp);
+    float a = probe_div(im, x, y, dp, fast, p[0], p[1], chan(p[4]), 10000.0f);
+    float b = probe_div(im, x, y, dp, fast, p[2], p[3], chan(p[5]), 10000.0f);
+    return a - b;
   }
   PD_HD static float eval(const ImageView<uint16_t, 4> &im, int x, int y, const float *p)
   {
-    float d = (float)im.at(x, y, 3);
-    if (!(d > 0.0f)) return 0.0f;
-    float a = probe(im, x, y, d, p[0], p[1], chan(p[4]));
-    float b = probe(im, x, y, d, p[2], p[3], chan(p[5]));
-    return a - b;
+    return eval_prep(im, x, y, prepare(im, x, y), p);
   }
 };
 
@@ -129,26 +207,51 @@ struct RGBDFeature {
 struct DemoS16Feature {
   using ImgType = uint16_t;
   using FeatType = int16_t;
+  using Prep = DivPrep;   // d = (float)(short)I(x), exact
   static constexpr int nChannels = 1;
   static constexpr int FeatDim = 2;
   static constexpr int kind = 3;
-  PD_HD static int16_t eval(const ImageView<uint16_t, 1> &im, int x, int y, const int16_t *p)
+  PD_HD static Prep prepare(const ImageView<uint16_t, 1> &im, int x, int y)
   {
-    int16_t depth = (int16_t)im.at(x, y, 0);
+    return div_prep((float)(int16_t)im.at(x, y, 0));
+  }
+  PD_HD static int16_t eval_prep(const ImageView<uint16_t, 1> &im, int x, int y, Prep dp, const int16_t *p)
+  {
+    const int16_t depth = (int16_t)dp.d;
     if (depth == 0) return 0;
-    float fd = (float)depth;
-    float qx = roundf((float)p[0] * 500.0f / fd);
-    float qy = roundf((float)p[1] * 500.0f / fd);
-    int16_t d2;
-    if (!offset_ok(qx) || !offset_ok(qy)) {
-      d2 = (int16_t)10000;
-    } else {
-      int ox = x + (int)qx, oy = y + (int)qy;
-      d2 = im.inside(ox, oy) ? (int16_t)im.at(ox, oy, 0) : (int16_t)10000;
-    }
+    // a negative depth (a wrapped uint16) takes the plain division
+    const bool fast = div_fast_ok(dp);
+    const int ox = x + offset_of(div_by((float)p[0] * 500.0f, dp, fast));
+    const int oy = y + offset_of(div_by((float)p[1] * 500.0f, dp, fast));
+    const bool in = (uint32_t)ox < (uint32_t)im.w && (uint32_t)oy < (uint32_t)im.h;
+    const int16_t v = (int16_t)im.at(clampi(ox, 0, im.w - 1), clampi(oy, 0, im.h - 1), 0);
+    const int16_t d2 = in ? v : (int16_t)10000;
     return (int16_t)(depth - (d2 ? d2 : (int16_t)10000));
   }
+  PD_HD static int16_t eval(const ImageView<uint16_t, 1> &im, int x, int y, const int16_t *p)
+  {
+    return eval_prep(im, x, y, prepare(im, x, y), p);
+  }
 };
+
+// Prep <-> the two 32-bit words (z, w) of the 16-byte sample record.
+template <typename P>
+PD_HD void pack_prep(P v, uint32_t &z, uint32_t &w)
+{
+  static_assert(sizeof(P) <= 8, "Prep must fit in 64 bits");
+  uint32_t t[2] = {0u, 0u};
+  memcpy(t, &v, sizeof(P));
+  z = t[0];
+  w = t[1];
+}
+template <typename P>
+PD_HD P unpack_prep(uint32_t z, uint32_t w)
+{
+  uint32_t t[2] = {z, w};
+  P v;
+  memcpy(&v, t, sizeof(P));
+  return v;
+}
 
 // Threshold padding value: compares "left" against every response, so a
 // padded slot never counts as "right" for a finite response.

@@ -140,6 +140,8 @@ int pd_candidate_tables(int device, int feature_kind, const pd_train_params *par
 /* Unit entry points (device evaluation of single steps). */
 int pd_md5rand(int device, uint32_t n, const uint32_t *seeds, uint32_t *out);
 int pd_log2_det(int device, uint32_t n, const float *x, float *y);
+/* q[i] = p[i] / d[i] through the feature hooks' per-pixel division path. */
+int pd_fast_div(int device, uint32_t n, const float *p, const float *d, float *q);
 int pd_feature_eval(int device, int feature_kind, const void *image, uint32_t w, uint32_t h, uint32_t n,
                     const int32_t *xy, const void *params, void *out);
 

@@ -107,6 +107,7 @@ def lib():
         "pd_candidate_tables": (I32, [I32, I32, C.POINTER(TrainParams), U32, U32, VP, VP, VP]),
         "pd_md5rand": (I32, [I32, U32, VP, VP]),
         "pd_log2_det": (I32, [I32, U32, VP, VP]),
+        "pd_fast_div": (I32, [I32, U32, VP, VP, VP]),
         "pd_feature_eval": (I32, [I32, I32, VP, U32, U32, U32, VP, VP, VP]),
         "pd_forest_create": (I32, [C.POINTER(VP), I32, I32, U32]),
         "pd_forest_destroy": (I32, [VP]),

@@ -162,13 +162,14 @@ double gain_f64(const uint32_t *left, const uint32_t *tot, uint32_t nC)
   return h - (nL / N * hL + nR / N * hR);
 }
 
+// compiled threshold-count buckets of the histogram kernel (exact for the
+// common 10 / 20 / 32; a larger bucket pads with thresholds no finite
+// response exceeds)
 int n_thr_bucket(uint32_t nT)
 {
-  if (nT <= 8) return 8;
-  if (nT <= 16) return 16;
-  if (nT <= 24) return 24;
-  if (nT <= 32) return 32;
-  if (nT <= 48) return 48;
+  static const int buckets[] = {8, 10, 16, 20, 24, 32, 48, 64};
+  for (int b : buckets)
+    if (nT <= (uint32_t)b) return b;
   return 64;
 }
 
@@ -237,7 +238,7 @@ struct Trainer final : TrainerBase {
   // training set
   uint32_t n_images = 0;
   uint64_t S = 0;
-  std::vector<uint2> h_recs;
+  std::vector<uint4> h_recs;
   std::vector<uint32_t> root_tot;
   DevBuf d_pixels, d_imgs, d_recs;
   // level state
@@ -294,15 +295,23 @@ struct Trainer final : TrainerBase {
         require(l >= 1 && l <= nC, "sampled pixel is unlabelled or its label exceeds n_classes");
         const uint32_t c = l - 1u;
         root_tot[c]++;  // perClassTotSamples at depth 1, _traverse_ts.hpp:526
-        h_recs[g] = make_uint2(i | (c << 24), (px % w) | ((px / w) << 16));
+        h_recs[g] = make_uint4(i | (c << 24), (px % w) | ((px / w) << 16), 0u, 0u);
       }
     }
     d_pixels.ensure(total_px * sizeof(I));
     HIPCHK(hipMemcpy(d_pixels.p, pixels, total_px * sizeof(I), hipMemcpyHostToDevice));
     d_imgs.ensure(ni * sizeof(ImgDesc));
     HIPCHK(hipMemcpy(d_imgs.p, imgs.data(), ni * sizeof(ImgDesc), hipMemcpyHostToDevice));
-    d_recs.ensure(std::max<uint64_t>(s_total, 1) * sizeof(uint2));
-    if (s_total) HIPCHK(hipMemcpy(d_recs.p, h_recs.data(), s_total * sizeof(uint2), hipMemcpyHostToDevice));
+    d_recs.ensure(std::max<uint64_t>(s_total, 1) * sizeof(uint4));
+    if (s_total) {
+      HIPCHK(hipMemcpy(d_recs.p, h_recs.data(), s_total * sizeof(uint4), hipMemcpyHostToDevice));
+      // per-sample prepared state (the functor's shared per-pixel part)
+      hipLaunchKernelGGL((k_prepare<F>), dim3((uint32_t)((s_total + 255) / 256)), dim3(256), 0, stream,
+                         d_pixels.as<I>(), d_imgs.as<ImgDesc>(), d_recs.as<uint4>(), (uint32_t)s_total);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipStreamSynchronize(stream));
+      HIPCHK(hipMemcpy(h_recs.data(), d_recs.p, s_total * sizeof(uint4), hipMemcpyDeviceToHost));
+    }
     n_images = ni;
     S = s_total;
   }
@@ -318,7 +327,7 @@ struct Trainer final : TrainerBase {
   }
 
   template <int NTP>
-  void launch_hist_t(uint32_t n_items, uint32_t item_base, const uint2 *act, const Bounds<F> &bnd, uint32_t tree,
+  void launch_hist_t(uint32_t n_items, uint32_t item_base, const uint4 *act, const Bounds<F> &bnd, uint32_t tree,
                      uint32_t nF, uint32_t nT)
   {
     const size_t lds = hist_lds_bytes<F, NTP>(nC);
@@ -332,13 +341,15 @@ struct Trainer final : TrainerBase {
     HIPCHK(hipGetLastError());
   }
 
-  void launch_hist(uint32_t n_items, uint32_t item_base, const uint2 *act, const Bounds<F> &bnd, uint32_t tree,
+  void launch_hist(uint32_t n_items, uint32_t item_base, const uint4 *act, const Bounds<F> &bnd, uint32_t tree,
                    uint32_t nF, uint32_t nT)
   {
     if (!n_items) return;
     switch (n_thr_bucket(nT)) {
       case 8: launch_hist_t<8>(n_items, item_base, act, bnd, tree, nF, nT); break;
+      case 10: launch_hist_t<10>(n_items, item_base, act, bnd, tree, nF, nT); break;
       case 16: launch_hist_t<16>(n_items, item_base, act, bnd, tree, nF, nT); break;
+      case 20: launch_hist_t<20>(n_items, item_base, act, bnd, tree, nF, nT); break;
       case 24: launch_hist_t<24>(n_items, item_base, act, bnd, tree, nF, nT); break;
       case 32: launch_hist_t<32>(n_items, item_base, act, bnd, tree, nF, nT); break;
       case 48: launch_hist_t<48>(n_items, item_base, act, bnd, tree, nF, nT); break;
@@ -348,7 +359,7 @@ struct Trainer final : TrainerBase {
 
   // Histograms of one slice of slots into d_H (slice-local slot index).
   // Returns the number of kernel launches.
-  uint32_t hist_slice(const std::vector<SlotInfo> &slots, size_t s0, size_t s1, const uint2 *act,
+  uint32_t hist_slice(const std::vector<SlotInfo> &slots, size_t s0, size_t s1, const uint4 *act,
                       const pd_train_params &P, const Bounds<F> &bnd, uint32_t tree, int mode, uint32_t ch,
                       const Event *ev_begin = nullptr, const Event *ev_end = nullptr)
   {
@@ -488,13 +499,13 @@ struct Trainer final : TrainerBase {
     d_tleft.ensure(N * sizeof(int32_t));
     d_tfeat.ensure(N * FD * sizeof(T));
     d_tthr.ensure(N * sizeof(T));
-    d_actA.ensure(S * sizeof(uint2));
-    d_actB.ensure(S * sizeof(uint2));
+    d_actA.ensure(S * sizeof(uint4));
+    d_actB.ensure(S * sizeof(uint4));
     d_slotA.ensure(S * sizeof(uint32_t));
     d_slotB.ensure(S * sizeof(uint32_t));
-    HIPCHK(hipMemcpyAsync(d_actA.p, d_recs.p, S * sizeof(uint2), hipMemcpyDeviceToDevice, stream));
+    HIPCHK(hipMemcpyAsync(d_actA.p, d_recs.p, S * sizeof(uint4), hipMemcpyDeviceToDevice, stream));
     HIPCHK(hipMemsetAsync(d_slotA.p, 0, S * sizeof(uint32_t), stream));
-    uint2 *act = d_actA.as<uint2>(), *act_next = d_actB.as<uint2>();
+    uint4 *act = d_actA.as<uint4>(), *act_next = d_actB.as<uint4>();
     uint32_t *aslot = d_slotA.as<uint32_t>(), *aslot_next = d_slotB.as<uint32_t>();
     uint64_t n_act = S;
 
@@ -659,23 +670,23 @@ struct Trainer final : TrainerBase {
       require(slot_of.count(nodes[q]) == 0, "duplicate node id");
       slot_of[nodes[q]] = q;
     }
-    std::vector<std::vector<uint2>> per(n_nodes);
+    std::vector<std::vector<uint4>> per(n_nodes);
     for (uint64_t s = 0; s < S; s++) {
       auto itn = slot_of.find(sample_node[s]);
       if (itn != slot_of.end()) per[itn->second].push_back(h_recs[s]);
     }
-    std::vector<uint2> act;
+    std::vector<uint4> act;
     std::vector<SlotInfo> slots;
     for (uint32_t q = 0; q < n_nodes; q++) {
       slots.push_back(SlotInfo{nodes[q], (uint32_t)act.size(), (uint32_t)per[q].size(), true});
       act.insert(act.end(), per[q].begin(), per[q].end());
     }
-    d_actA.ensure(std::max<size_t>(act.size(), 1) * sizeof(uint2));
+    d_actA.ensure(std::max<size_t>(act.size(), 1) * sizeof(uint4));
     if (!act.empty())
-      HIPCHK(hipMemcpyAsync(d_actA.p, act.data(), act.size() * sizeof(uint2), hipMemcpyHostToDevice, stream));
+      HIPCHK(hipMemcpyAsync(d_actA.p, act.data(), act.size() * sizeof(uint4), hipMemcpyHostToDevice, stream));
     const Bounds<F> bnd = make_bounds<F>(P);
     const uint32_t ch = pick_chunk(act.size(), (P.n_features + 63) / 64);
-    hist_slice(slots, 0, slots.size(), d_actA.as<uint2>(), P, bnd, tree_id, PD_MODE_LEVEL_BATCHED, ch);
+    hist_slice(slots, 0, slots.size(), d_actA.as<uint4>(), P, bnd, tree_id, PD_MODE_LEVEL_BATCHED, ch);
     const size_t per_n = (size_t)nC * P.n_thresholds * P.n_features;
     HIPCHK(hipMemcpyAsync(H_out, d_H.p, per_n * n_nodes * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
@@ -1134,6 +1145,24 @@ int pd_md5rand(int device, uint32_t n, const uint32_t *seeds, uint32_t *out)
     hipLaunchKernelGGL(k_md5, dim3((n + 255) / 256), dim3(256), 0, nullptr, di.as<uint4>(), dout.as<uint4>(), n);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpy(out, dout.p, (size_t)n * 16, hipMemcpyDeviceToHost));
+  });
+}
+
+int pd_fast_div(int device, uint32_t n, const float *p, const float *d, float *q)
+{
+  return guarded([&] {
+    require(n >= 1 && p && d && q, "bad division arguments");
+    set_device(device);
+    DevBuf dp, dd, dq;
+    dp.ensure((size_t)n * 4);
+    dd.ensure((size_t)n * 4);
+    dq.ensure((size_t)n * 4);
+    HIPCHK(hipMemcpy(dp.p, p, (size_t)n * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dd.p, d, (size_t)n * 4, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_div, dim3((n + 255) / 256), dim3(256), 0, nullptr, dp.as<float>(), dd.as<float>(),
+                       dq.as<float>(), n);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpy(q, dq.p, (size_t)n * 4, hipMemcpyDeviceToHost));
   });
 }
 

@@ -50,40 +50,89 @@ struct Bounds {
   typename F::FeatType tlo, thi;
 };
 
-// sample record: x = image | class << 24, y = px_x | px_y << 16
-__device__ __forceinline__ uint32_t rec_img(uint2 r) { return r.x & 0xffffffu; }
-__device__ __forceinline__ uint32_t rec_cls(uint2 r) { return r.x >> 24; }
-__device__ __forceinline__ int rec_x(uint2 r) { return (int)(r.y & 0xffffu); }
-__device__ __forceinline__ int rec_y(uint2 r) { return (int)(r.y >> 16); }
+// sample record (16 B): x = image | class << 24, y = px_x | px_y << 16,
+// (z, w) = the functor's prepared per-pixel state (Prep bits)
+__device__ __forceinline__ uint32_t rec_img(uint4 r) { return r.x & 0xffffffu; }
+__device__ __forceinline__ uint32_t rec_cls(uint4 r) { return r.x >> 24; }
+__device__ __forceinline__ int rec_x(uint4 r) { return (int)(r.y & 0xffffu); }
+__device__ __forceinline__ int rec_y(uint4 r) { return (int)(r.y >> 16); }
 
-constexpr int HIST_THREADS = 256;
-constexpr int HIST_WAVES = 4;
+constexpr int HIST_THREADS = 512;   // 8 wave64s share one 64-candidate tile
 
 // words per (class, lane) of the 16-bit rank bins for ranks 0..NTP
 template <int NTP>
 constexpr int rank_words() { return (NTP + 2) / 2; }
 
+// power-of-two slot count of the per-lane sorted threshold list
+template <int NTP>
+constexpr int sorted_slots() { return NTP <= 8 ? 8 : (NTP <= 16 ? 16 : (NTP <= 32 ? 32 : 64)); }
+
 template <class F, int NTP>
 constexpr size_t hist_lds_bytes(uint32_t nC)
 {
   return (size_t)nC * rank_words<NTP>() * 64 * 4 +                         // rank bins
+         (size_t)sorted_slots<NTP>() * 64 * sizeof(typename F::FeatType) + // sorted thresholds
          (size_t)(F::FeatDim + NTP) * 64 * sizeof(typename F::FeatType) +  // staging
          (size_t)NTP * 64;                                                   // ranks L_t
+}
+
+// Fill the Prep word of every sample record.
+template <class F>
+__global__ void k_prepare(const typename F::ImgType *__restrict__ pixels, const ImgDesc *__restrict__ imgs,
+                          uint4 *__restrict__ recs, uint32_t n)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint4 r = recs[i];
+  const ImgDesc d = imgs[rec_img(r)];
+  ImageView<typename F::ImgType, F::nChannels> view{pixels + d.off, (int)d.w, (int)d.h};
+  pack_prep<typename F::Prep>(F::prepare(view, rec_x(r), rec_y(r)), r.z, r.w);
+  recs[i] = r;
+}
+
+// One (sample, candidate) pair: response, rank r = #{t : !(v <= thr_t)} by a
+// branch-free binary search over the lane's sorted thresholds (the
+// predicate is true exactly on a prefix of the sorted list; a NaN response
+// makes it true everywhere), then one LDS add into bin [class][r].
+template <class F, int NTP>
+__device__ __forceinline__ void hist_sample(const typename F::ImgType *__restrict__ pixels,
+                                            const ImgDesc *__restrict__ imgs, uint4 r,
+                                            const typename F::FeatType (&p)[F::FeatDim],
+                                            const typename F::FeatType *sth, uint32_t nT, bool fv,
+                                            uint32_t lane, uint32_t *bins)
+{
+  using T = typename F::FeatType;
+  constexpr int RW = rank_words<NTP>();
+  constexpr int NS = sorted_slots<NTP>();
+  const ImgDesc d = imgs[rec_img(r)];
+  ImageView<typename F::ImgType, F::nChannels> view{pixels + d.off, (int)d.w, (int)d.h};
+  const T v = F::eval_prep(view, rec_x(r), rec_y(r), unpack_prep<typename F::Prep>(r.z, r.w), p);
+  uint32_t pos = 0;
+#pragma unroll
+  for (int step = NS / 2; step >= 1; step >>= 1) {
+    const T s = sth[(pos + step - 1) * 64u + lane];
+    pos += (v <= s) ? 0u : (uint32_t)step;
+  }
+  const uint32_t rk = pos < nT ? pos : nT;
+  if (fv) atomicAdd(&bins[(rec_cls(r) * RW + (rk >> 1)) * 64u + lane], 1u << ((rk & 1u) * 16u));
 }
 
 template <class F, int NTP>
 __global__ __launch_bounds__(HIST_THREADS) void k_hist(
     const typename F::ImgType *__restrict__ pixels, const ImgDesc *__restrict__ imgs,
-    const uint2 *__restrict__ recs, const HistItem *__restrict__ items, uint32_t item_base,
+    const uint4 *__restrict__ recs, const HistItem *__restrict__ items, uint32_t item_base,
     const int32_t *__restrict__ slot_node, const uint8_t *__restrict__ slot_atomic, Bounds<F> bnd,
     uint32_t tree, uint32_t nF, uint32_t nT, uint32_t nC, uint32_t *__restrict__ H)
 {
   using T = typename F::FeatType;
   constexpr int FD = F::FeatDim;
   constexpr int RW = rank_words<NTP>();
+  constexpr uint32_t NW = HIST_THREADS / 64;
+  constexpr int NS = sorted_slots<NTP>();
   extern __shared__ __align__(16) uint32_t lds[];
   uint32_t *bins = lds;
-  T *sp = (T *)(bins + (size_t)nC * RW * 64);
+  T *sth = (T *)(bins + (size_t)nC * RW * 64);
+  T *sp = sth + NS * 64;
   T *st = sp + FD * 64;
   uint8_t *sl = (uint8_t *)(st + NTP * 64);
 
@@ -113,49 +162,53 @@ __global__ __launch_bounds__(HIST_THREADS) void k_hist(
     }
   }
   __syncthreads();
-  // ranks L_t = #{t' < nT : thr_t' < thr_t}, one (t, f) pair per thread step
-  for (uint32_t t = tid >> 6; t < (uint32_t)NTP; t += HIST_WAVES) {
-    T th = st[t * 64 + lane];
-    uint32_t L = 0;
-#pragma unroll
-    for (int t2 = 0; t2 < NTP; t2++) L += ((uint32_t)t2 < nT && st[t2 * 64 + lane] < th) ? 1u : 0u;
+  // Per (t, f): rank L_t = #{t' < nT : thr_t' < thr_t} (used by the flush)
+  // and the stable sort position (ties by index) that scatters the lane's
+  // thresholds into sorted order; slots nT..NS-1 hold the padding value.
+  for (uint32_t t = tid >> 6; t < (uint32_t)NS; t += NW) {
+    if (t >= nT) {
+      sth[t * 64 + lane] = pad_threshold<T>();
+      continue;
+    }
+    const T th = st[t * 64 + lane];
+    uint32_t L = 0, P = 0;
+#pragma unroll 4
+    for (int t2 = 0; t2 < NTP; t2++) {
+      const T o = st[t2 * 64 + lane];
+      const bool live = (uint32_t)t2 < nT;
+      L += (live && o < th) ? 1u : 0u;
+      P += (live && (o < th || (o == th && (uint32_t)t2 < t))) ? 1u : 0u;
+    }
     sl[t * 64 + lane] = (uint8_t)L;
+    sth[P * 64 + lane] = th;
   }
   T p[FD];
 #pragma unroll
   for (int j = 0; j < FD; j++) p[j] = sp[j * 64 + lane];
-  T thr[NTP];
-#pragma unroll
-  for (int t = 0; t < NTP; t++) thr[t] = st[t * 64 + lane];
+  __syncthreads();
 
-  // Sample loop.  r(v) = #{t : !(v <= thr_t)} is #{t : thr_t < v} for a
-  // finite v and nT for NaN (padded slots never count for finite v).
-  // v <= thr_t  <=>  r(v) <= L_t:  if v <= thr_t every thr below v is below
-  // thr_t; if v > thr_t then thr_t and every thr <= thr_t are below v, so
-  // r(v) >= L_t + 1.
+  // Sample loop, two samples in flight per wave.  r(v) = #{t : !(v <= thr_t)}
+  // is #{t : thr_t < v} for a finite v and >= nT for NaN (padded slots never
+  // count for finite v).  v <= thr_t  <=>  r(v) <= L_t:  if v <= thr_t every
+  // thr below v is below thr_t; if v > thr_t then thr_t and every thr <= thr_t
+  // are below v, so r(v) >= L_t + 1.
   const uint32_t end = it.begin + it.count;
-  for (uint32_t i = it.begin + wave; i < end; i += HIST_WAVES) {
-    const uint2 r = recs[i];
-    const ImgDesc d = imgs[rec_img(r)];
-    ImageView<typename F::ImgType, F::nChannels> view{pixels + d.off, (int)d.w, (int)d.h};
-    const T v = F::eval(view, rec_x(r), rec_y(r), p);
-    uint32_t rk = 0;
-#pragma unroll
-    for (int t = 0; t < NTP; t++) rk += (v <= thr[t]) ? 0u : 1u;
-    rk = rk < nT ? rk : nT;
-    if (fv) atomicAdd(&bins[(rec_cls(r) * RW + (rk >> 1)) * 64u + lane], 1u << ((rk & 1u) * 16u));
+  uint32_t i = it.begin + wave;
+  for (; i + NW < end; i += 2 * NW) {
+    const uint4 ra = recs[i];
+    const uint4 rb = recs[i + NW];
+    hist_sample<F, NTP>(pixels, imgs, ra, p, sth, nT, fv, lane, bins);
+    hist_sample<F, NTP>(pixels, imgs, rb, p, sth, nT, fv, lane, bins);
   }
+  if (i < end) hist_sample<F, NTP>(pixels, imgs, recs[i], p, sth, nT, fv, lane, bins);
   __syncthreads();
 
   // Flush: per class (classes strided over waves) turn this lane's rank bins
   // into prefix counts in place, then count(c, t) = prefix[L_t].
-  uint8_t L[NTP];
-#pragma unroll
-  for (int t = 0; t < NTP; t++) L[t] = sl[t * 64 + lane];
   const bool atom = slot_atomic[it.slot] != 0;
   const size_t per_class = (size_t)nT * nF;
   uint32_t *Hs = H + (size_t)it.slot * nC * per_class;
-  for (uint32_t c = wave; c < nC; c += HIST_WAVES) {
+  for (uint32_t c = wave; c < nC; c += NW) {
     uint32_t *b = bins + (size_t)c * RW * 64u + lane;
     uint32_t run = 0;
 #pragma unroll
@@ -171,7 +224,7 @@ __global__ __launch_bounds__(HIST_THREADS) void k_hist(
 #pragma unroll
     for (int t = 0; t < NTP; t++) {
       if ((uint32_t)t >= nT) break;
-      const uint32_t l = L[t];
+      const uint32_t l = sl[t * 64 + lane];
       const uint32_t val = (b[(l >> 1) * 64u] >> ((l & 1u) * 16u)) & 0xffffu;
       if (atom) {
         if (val) atomicAdd(out + (size_t)t * nF, val);
@@ -347,21 +400,21 @@ __device__ __forceinline__ uint32_t aggregated_append(uint32_t *ctr, int key, bo
 
 template <class F>
 __global__ __launch_bounds__(256) void k_route(const typename F::ImgType *__restrict__ pixels,
-                                               const ImgDesc *__restrict__ imgs, const uint2 *__restrict__ recs,
+                                               const ImgDesc *__restrict__ imgs, const uint4 *__restrict__ recs,
                                                const uint32_t *__restrict__ rec_slot, uint32_t n,
                                                const int32_t *__restrict__ slot_node,
                                                const int32_t *__restrict__ tree_left,
                                                const typename F::FeatType *__restrict__ tree_feat,
                                                const typename F::FeatType *__restrict__ tree_thr,
                                                const int32_t *__restrict__ slot_next, uint32_t *__restrict__ cursor,
-                                               uint32_t *__restrict__ child_count, uint2 *__restrict__ out_recs,
+                                               uint32_t *__restrict__ child_count, uint4 *__restrict__ out_recs,
                                                uint32_t *__restrict__ out_slot, uint32_t out_cap)
 {
   using T = typename F::FeatType;
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   bool live = i < n;
   uint32_t slot = 0, side = 0;
-  uint2 r = make_uint2(0u, 0u);
+  uint4 r = make_uint4(0u, 0u, 0u, 0u);
   if (live) {
     slot = rec_slot[i];
     const int32_t node = slot_node[slot];
@@ -375,7 +428,7 @@ __global__ __launch_bounds__(256) void k_route(const typename F::ImgType *__rest
       T p[F::FeatDim];
 #pragma unroll
       for (int j = 0; j < F::FeatDim; j++) p[j] = tree_feat[(size_t)node * F::FeatDim + j];
-      const T v = F::eval(view, rec_x(r), rec_y(r), p);
+      const T v = F::eval_prep(view, rec_x(r), rec_y(r), unpack_prep<typename F::Prep>(r.z, r.w), p);
       side = (v <= tree_thr[node]) ? 0u : 1u;
     }
   }
@@ -439,6 +492,7 @@ __global__ __launch_bounds__(256) void k_predict(
   const int x = (int)(px % fd.w), y = (int)(px / fd.w);
   const bool on = masks ? masks[fd.mask_off + px] != 0 : true;
   ImageView<typename F::ImgType, F::nChannels> view{pixels + fd.img_off, (int)fd.w, (int)fd.h};
+  const typename F::Prep prep = F::prepare(view, x, y);
   float acc[NCM];
 #pragma unroll
   for (int c = 0; c < NCM; c++) acc[c] = 0.0f;
@@ -450,7 +504,7 @@ __global__ __launch_bounds__(256) void k_predict(
       for (uint32_t s = 0; s < max_steps; s++) {
         const NodeRec<T, F::FeatDim> rec = tr[n];
         if (rec.left < 0) break;
-        const T v = F::eval(view, x, y, rec.p);
+        const T v = F::eval_prep(view, x, y, prep, rec.p);
         n = rec.left + ((v <= rec.thr) ? 0 : 1);
       }
       if (out_post) {
@@ -501,6 +555,14 @@ __global__ void k_feature_eval(const typename F::ImgType *__restrict__ pixels, u
   if (i >= n) return;
   ImageView<typename F::ImgType, F::nChannels> view{pixels, (int)w, (int)h};
   out[i] = F::eval(view, xy[2 * i], xy[2 * i + 1], params + (size_t)i * F::FeatDim);
+}
+
+__global__ void k_div(const float *__restrict__ p, const float *__restrict__ d, float *__restrict__ q, uint32_t n)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const DivPrep dp = div_prep(d[i]);
+  q[i] = div_by(p[i], dp, div_fast_ok(dp));
 }
 
 __global__ void k_log2_det(const float *__restrict__ x, float *__restrict__ y, uint32_t n)

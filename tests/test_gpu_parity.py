@@ -40,6 +40,28 @@ def test_log2_det_device_bit_exact(gpu):
     assert y.tobytes() == want.tobytes()
 
 
+def test_fast_division_device(gpu):
+    """The hooks' per-pixel division equals IEEE float32 division."""
+    rng = np.random.default_rng(11)
+    n = 2_000_000
+    d = np.concatenate([rng.integers(1, 65536, n // 2).astype(np.float32),
+                        np.exp2(rng.uniform(-16, 16, n // 2)).astype(np.float32)])
+    mag = np.exp2(rng.uniform(-60, 60, n)).astype(np.float32)
+    p = (mag * rng.choice([-1.0, 1.0], n)).astype(np.float32)
+    p[:1000] = 0.0
+    p[1000:200000] = rng.integers(-200000, 200000, 199000).astype(np.float32)  # integral numerators
+    q = np.zeros(n, np.float32)
+    L.check(L.lib().pd_fast_div(0, n, L.ptr(p), L.ptr(d), L.ptr(q)))
+    want = (p / d).astype(np.float32)
+    assert np.array_equal(q, want)
+    # outside the fast range the hooks use the plain division
+    d2 = np.array([1e-30, 1e30, 2.0 ** 17, 2.0 ** -17, 3.0], np.float32)
+    p2 = np.array([1.0, 5.0, 7.0, 1e-3, 1e-40], np.float32)
+    q2 = np.zeros(5, np.float32)
+    L.check(L.lib().pd_fast_div(0, 5, L.ptr(p2), L.ptr(d2), L.ptr(q2)))
+    assert np.array_equal(q2[:4], (p2 / d2)[:4].astype(np.float32))
+
+
 def _rand_image(kind, rng, w, h):
     if kind == pa.PD_FEAT_DEPTH_F32:
         im = rng.uniform(0.5, 6.0, (h, w)).astype(np.float32)
@@ -146,8 +168,9 @@ def test_split_search_device(gpu):
         tot = rng.integers(1, 400, (nodes, nC)).astype(np.uint32)
         H = np.stack([np.stack([rng.integers(0, t + 1, (nT, nF)) for t in tot[q]]) for q in range(nodes)])
         H = H.astype(np.uint32)
-        # plant exact ties: the best pure split at two places
-        H[0, :, nT - 1, 0] = 0
+        # node 0: every pair sends all samples right (gain 0) except two
+        # identical informative pairs -> an exact tie between them
+        H[0] = 0
         H[0, 0, nT - 1, 0] = tot[0, 0]
         H[0, :, 0, nF - 1] = H[0, :, nT - 1, 0]
         bk = np.zeros(nodes, dtype=np.uint32)
@@ -234,7 +257,7 @@ def test_train_end_depth_and_root_gate(gpu):
     tr.train(tree, ts, p, 1, 4)            # train only 3 levels of a depth-8 tree
     out = orc.train_tree(orc.Tset(0, ts), p, ts.priors, 0, 8, end_depth=4)
     assert np.array_equal(tree.left_children, out["left_children"])
-    assert (tree.left_children[7:] == -2).all()
+    assert (tree.left_children[15:] == -2).all()   # levels 5.. do not exist
     p.per_leaf_samples_thr = 10000         # root holds fewer samples: empty histogram, no split
     tr.train(tree, ts, p, 1, 8)
     assert (tree.left_children == -2).all()
