@@ -98,20 +98,18 @@ PD_HD int offset_of(float q)
   return fabsf(q) < 0.5f ? 0 : n;
 }
 
-PD_HD int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
-
 // Read of channel `ch` at x + round(px/d), y + round(py/d), or `bg` when the
-// offset leaves the image.  Branch-free: the address is clamped into the
-// image and the background selected afterwards (same value as testing first).
+// offset leaves the image.  Only the load is predicated, so lanes whose
+// probe leaves the image issue no memory access.
 template <typename ImgT, int NCH>
 PD_HD float probe_div(const ImageView<ImgT, NCH> &im, int x, int y, const DivPrep &dp, bool fast, float px,
                       float py, int ch, float bg)
 {
   const int ox = x + offset_of(div_by(px, dp, fast));
   const int oy = y + offset_of(div_by(py, dp, fast));
-  const bool in = (uint32_t)ox < (uint32_t)im.w && (uint32_t)oy < (uint32_t)im.h;
-  const float v = (float)im.at(clampi(ox, 0, im.w - 1), clampi(oy, 0, im.h - 1), ch);
-  return in ? v : bg;
+  float v = bg;
+  if ((uint32_t)ox < (uint32_t)im.w && (uint32_t)oy < (uint32_t)im.h) v = (float)im.at(ox, oy, ch);
+  return v;
 }
 
 // C1: float depth in metres.  f = I(x + round(u/d(x))) - I(x + round(v/d(x))),
@@ -223,9 +221,8 @@ struct DemoS16Feature {
     const bool fast = div_fast_ok(dp);
     const int ox = x + offset_of(div_by((float)p[0] * 500.0f, dp, fast));
     const int oy = y + offset_of(div_by((float)p[1] * 500.0f, dp, fast));
-    const bool in = (uint32_t)ox < (uint32_t)im.w && (uint32_t)oy < (uint32_t)im.h;
-    const int16_t v = (int16_t)im.at(clampi(ox, 0, im.w - 1), clampi(oy, 0, im.h - 1), 0);
-    const int16_t d2 = in ? v : (int16_t)10000;
+    int16_t d2 = (int16_t)10000;
+    if ((uint32_t)ox < (uint32_t)im.w && (uint32_t)oy < (uint32_t)im.h) d2 = (int16_t)im.at(ox, oy, 0);
     return (int16_t)(depth - (d2 ? d2 : (int16_t)10000));
   }
   PD_HD static int16_t eval(const ImageView<uint16_t, 1> &im, int x, int y, const int16_t *p)

@@ -477,7 +477,8 @@ int or_hist_nodes(const or_tset *ts, const or_params *pr, uint32_t nC, uint32_t 
     unsigned char params[64], thr[512];
     or_feature_params(kind, tree_id, node, f, pr->flo, pr->fhi, params);
     or_thresholds(kind, tree_id, node, f, nT, pr->thrlo, pr->thrhi, thr);
-    uint32_t *Hs = H + per * sl;
+    /* this candidate's counts [c][t], private to the thread */
+    uint32_t *loc = (uint32_t *)calloc((size_t)nC * nT, sizeof(uint32_t));
     size_t g = 0;
     for (uint32_t i = 0; i < ts->n_images; i++) {
       or_image im = image_of(ts, i);
@@ -489,10 +490,13 @@ int or_hist_nodes(const or_tset *ts, const or_params *pr, uint32_t nC, uint32_t 
         int x = (int)(px % (uint32_t)im.w), y = (int)(px / (uint32_t)im.w);
         uint32_t c = (uint32_t)lab[px] - 1u;
         double v = or_feature(kind, &im, x, y, params);
-        for (uint32_t t = 0; t < nT; t++)
-          Hs[(size_t)c * nT * nF + (size_t)t * nF + f] += (uint32_t)goes_left(kind, v, thr, t);
+        for (uint32_t t = 0; t < nT; t++) loc[(size_t)c * nT + t] += (uint32_t)goes_left(kind, v, thr, t);
       }
     }
+    uint32_t *Hs = H + per * sl;
+    for (uint32_t c = 0; c < nC; c++)
+      for (uint32_t t = 0; t < nT; t++) Hs[(size_t)c * nT * nF + (size_t)t * nF + f] = loc[(size_t)c * nT + t];
+    free(loc);
   }
   (void)fd; (void)fsz;
   free(slot);

@@ -11,6 +11,8 @@
 #include "../../include/padenti_hip.h"
 #include "pd_kernels.hpp"
 
+#include <hipcub/hipcub.hpp>
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -245,6 +247,7 @@ struct Trainer final : TrainerBase {
   DevBuf d_actA, d_actB, d_slotA, d_slotB;
   DevBuf d_H, d_items, d_slot_node, d_slot_atomic, d_zero, d_tot, d_best, d_res;
   DevBuf d_tleft, d_tfeat, d_tthr, d_rslot_node, d_slot_next, d_cursor, d_ccount;
+  DevBuf d_flags, d_rank, d_sbegin, d_scan_tmp;
   uint32_t cu_count = 256;
   size_t lds_limit = 65536;
 
@@ -599,7 +602,7 @@ struct Trainer final : TrainerBase {
       // _impl_init.hpp:343-355), in node order
       std::vector<SlotInfo> next;
       std::vector<int32_t> snext(2 * slots.size(), -1), rnode(slots.size());
-      std::vector<uint32_t> cursor;
+      std::vector<uint32_t> nbegin;   // first entry of each next slot
       uint32_t run = 0;
       for (size_t q = 0; q < slots.size(); q++) {
         rnode[q] = slots[q].node;
@@ -610,32 +613,51 @@ struct Trainer final : TrainerBase {
           if (cnt > thr) {
             snext[2 * q + side] = (int32_t)next.size();
             next.push_back(SlotInfo{ch_node, run, cnt, true});
-            cursor.push_back(run);
+            nbegin.push_back(run);
             run += cnt;
           }
         }
       }
       const uint32_t nq = (uint32_t)slots.size();
+      std::vector<uint32_t> sbegin(nq + 1);
+      for (uint32_t q = 0; q < nq; q++) sbegin[q] = slots[q].begin;
+      sbegin[nq] = (uint32_t)n_act;
       d_rslot_node.ensure(nq * sizeof(int32_t));
       d_slot_next.ensure(2 * (size_t)nq * sizeof(int32_t));
       d_ccount.ensure(2 * (size_t)nq * sizeof(uint32_t));
-      d_cursor.ensure(std::max<size_t>(cursor.size(), 1) * sizeof(uint32_t));
+      d_cursor.ensure(std::max<size_t>(nbegin.size(), 1) * sizeof(uint32_t));
+      d_sbegin.ensure((nq + 1) * sizeof(uint32_t));
+      d_flags.ensure((n_act + 1) * sizeof(unsigned long long));
+      d_rank.ensure((n_act + 1) * sizeof(unsigned long long));
       HIPCHK(hipMemcpyAsync(d_rslot_node.p, rnode.data(), nq * sizeof(int32_t), hipMemcpyHostToDevice, stream));
       HIPCHK(hipMemcpyAsync(d_slot_next.p, snext.data(), 2 * (size_t)nq * sizeof(int32_t), hipMemcpyHostToDevice,
                             stream));
-      if (!cursor.empty())
-        HIPCHK(hipMemcpyAsync(d_cursor.p, cursor.data(), cursor.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+      HIPCHK(hipMemcpyAsync(d_sbegin.p, sbegin.data(), (nq + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+      if (!nbegin.empty())
+        HIPCHK(hipMemcpyAsync(d_cursor.p, nbegin.data(), nbegin.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
                               stream));
-      HIPCHK(hipMemsetAsync(d_ccount.p, 0, 2 * (size_t)nq * sizeof(uint32_t), stream));
+      size_t tmp_bytes = 0;
+      HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, d_flags.as<unsigned long long>(),
+                                              d_rank.as<unsigned long long>(), (int)(n_act + 1), stream));
+      d_scan_tmp.ensure(tmp_bytes);
       HIPCHK(hipEventRecord(e3.e, stream));
+      const uint32_t nb = (uint32_t)((n_act + 1 + 255) / 256);
+      hipLaunchKernelGGL((k_route_flags<F>), dim3(nb), dim3(256), 0, stream, d_pixels.as<I>(), d_imgs.as<ImgDesc>(),
+                         act, aslot, (uint32_t)n_act, d_rslot_node.as<int32_t>(), d_tleft.as<int32_t>(),
+                         d_tfeat.as<T>(), d_tthr.as<T>(), d_flags.as<unsigned long long>());
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipcub::DeviceScan::ExclusiveSum(d_scan_tmp.p, tmp_bytes, d_flags.as<unsigned long long>(),
+                                              d_rank.as<unsigned long long>(), (int)(n_act + 1), stream));
       if (n_act) {
-        hipLaunchKernelGGL((k_route<F>), dim3((uint32_t)((n_act + 255) / 256)), dim3(256), 0, stream,
-                           d_pixels.as<I>(), d_imgs.as<ImgDesc>(), act, aslot, (uint32_t)n_act,
-                           d_rslot_node.as<int32_t>(), d_tleft.as<int32_t>(), d_tfeat.as<T>(), d_tthr.as<T>(),
-                           d_slot_next.as<int32_t>(), d_cursor.as<uint32_t>(), d_ccount.as<uint32_t>(), act_next,
+        hipLaunchKernelGGL(k_route_scatter, dim3((uint32_t)((n_act + 255) / 256)), dim3(256), 0, stream, act, aslot,
+                           (uint32_t)n_act, d_flags.as<unsigned long long>(), d_rank.as<unsigned long long>(),
+                           d_sbegin.as<uint32_t>(), d_slot_next.as<int32_t>(), d_cursor.as<uint32_t>(), act_next,
                            aslot_next, (uint32_t)S);
         HIPCHK(hipGetLastError());
       }
+      hipLaunchKernelGGL(k_route_counts, dim3((nq + 255) / 256), dim3(256), 0, stream, d_rank.as<unsigned long long>(),
+                         d_sbegin.as<uint32_t>(), nq, d_ccount.as<uint32_t>());
+      HIPCHK(hipGetLastError());
       HIPCHK(hipEventRecord(e4.e, stream));
       std::vector<uint32_t> ccount(2 * (size_t)nq);
       HIPCHK(hipMemcpyAsync(ccount.data(), d_ccount.p, ccount.size() * sizeof(uint32_t), hipMemcpyDeviceToHost,

@@ -113,6 +113,9 @@ __device__ __forceinline__ void hist_sample(const typename F::ImgType *__restric
     const T s = sth[(pos + step - 1) * 64u + lane];
     pos += (v <= s) ? 0u : (uint32_t)step;
   }
+  // the steps reach at most NS-1: one more test of slot `pos` (the first
+  // slot whose predicate can be false) adds the last one when all NS are true
+  if (NS == NTP) pos += (v <= sth[pos * 64u + lane]) ? 0u : 1u;
   const uint32_t rk = pos < nT ? pos : nT;
   if (fv) atomicAdd(&bins[(rec_cls(r) * RW + (rk >> 1)) * 64u + lane], 1u << ((rk & 1u) * 16u));
 }
@@ -372,77 +375,83 @@ __global__ void k_finalize(const uint32_t *__restrict__ H, const unsigned long l
 
 // ---------------------------------------------------------------------------
 // Routing of one level (replaces re-running predict.cl from the root at
-// every level, cl_tree_trainer_impl_traverse_ts.hpp:211-244): each active
-// sample of a split node moves to child leftChild + (v <= thr ? 0 : 1) and is
-// appended to that child's run of the next active list if the child is in
-// the next frontier.  Appends are wave-aggregated (one atomic per distinct
-// destination per wave).  child_count[2*slot+side] counts every routed
-// sample so the host can check it against the split's class counts.
+// every level, cl_tree_trainer_impl_traverse_ts.hpp:211-244).  Order
+// preserving: k_route_flags evaluates each active sample of a split node
+// and writes a left/right flag pair; an exclusive scan of the pairs gives
+// every sample its rank among the samples of its node going the same way;
+// k_route_scatter appends it to its child's run of the next active list.
+// A node's samples therefore stay in (image, pixel) order at every level.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t aggregated_append(uint32_t *ctr, int key, bool active)
-{
-  // returns this lane's position; lanes with equal key share one atomic
-  uint64_t pending = __ballot(active);
-  uint32_t pos = 0;
-  const uint32_t lane = threadIdx.x & 63u;
-  while (pending) {
-    const int leader = __ffsll((unsigned long long)pending) - 1;
-    const int lkey = __shfl(key, leader, 64);
-    const uint64_t grp = __ballot(active && key == lkey) & pending;
-    uint32_t base = 0;
-    if ((int)lane == leader) base = atomicAdd(ctr + lkey, (uint32_t)__popcll(grp));
-    base = __shfl(base, leader, 64);
-    if ((grp >> lane) & 1ull) pos = base + (uint32_t)__popcll(grp & ((1ull << lane) - 1ull));
-    pending &= ~grp;
-  }
-  return pos;
-}
-
 template <class F>
-__global__ __launch_bounds__(256) void k_route(const typename F::ImgType *__restrict__ pixels,
-                                               const ImgDesc *__restrict__ imgs, const uint4 *__restrict__ recs,
-                                               const uint32_t *__restrict__ rec_slot, uint32_t n,
-                                               const int32_t *__restrict__ slot_node,
-                                               const int32_t *__restrict__ tree_left,
-                                               const typename F::FeatType *__restrict__ tree_feat,
-                                               const typename F::FeatType *__restrict__ tree_thr,
-                                               const int32_t *__restrict__ slot_next, uint32_t *__restrict__ cursor,
-                                               uint32_t *__restrict__ child_count, uint4 *__restrict__ out_recs,
-                                               uint32_t *__restrict__ out_slot, uint32_t out_cap)
+__global__ __launch_bounds__(256) void k_route_flags(const typename F::ImgType *__restrict__ pixels,
+                                                     const ImgDesc *__restrict__ imgs, const uint4 *__restrict__ recs,
+                                                     const uint32_t *__restrict__ rec_slot, uint32_t n,
+                                                     const int32_t *__restrict__ slot_node,
+                                                     const int32_t *__restrict__ tree_left,
+                                                     const typename F::FeatType *__restrict__ tree_feat,
+                                                     const typename F::FeatType *__restrict__ tree_thr,
+                                                     unsigned long long *__restrict__ flags)
 {
   using T = typename F::FeatType;
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  bool live = i < n;
-  uint32_t slot = 0, side = 0;
-  uint4 r = make_uint4(0u, 0u, 0u, 0u);
-  if (live) {
-    slot = rec_slot[i];
+  if (i > n) return;
+  unsigned long long fl = 0ull;   // (left) | (right << 32); entry n is the scan's total
+  if (i < n) {
+    const uint32_t slot = rec_slot[i];
     const int32_t node = slot_node[slot];
-    const int32_t lc = tree_left[node];
-    if (lc < 0) {
-      live = false;
-    } else {
-      r = recs[i];
+    if (tree_left[node] >= 0) {
+      const uint4 r = recs[i];
       const ImgDesc d = imgs[rec_img(r)];
       ImageView<typename F::ImgType, F::nChannels> view{pixels + d.off, (int)d.w, (int)d.h};
       T p[F::FeatDim];
 #pragma unroll
       for (int j = 0; j < F::FeatDim; j++) p[j] = tree_feat[(size_t)node * F::FeatDim + j];
       const T v = F::eval_prep(view, rec_x(r), rec_y(r), unpack_prep<typename F::Prep>(r.z, r.w), p);
-      side = (v <= tree_thr[node]) ? 0u : 1u;
+      fl = (v <= tree_thr[node]) ? 1ull : (1ull << 32);
     }
   }
-  const int ckey = (int)(2u * slot + side);
-  (void)aggregated_append(child_count, ckey, live);
-  const int ns = live ? slot_next[ckey] : -1;
-  const bool go = live && ns >= 0;
-  const uint32_t pos = aggregated_append(cursor, ns < 0 ? 0 : ns, go);
-  // out_cap bounds the write even if routing disagreed with the split
-  // counts; the host compares child_count with them and reports it.
-  if (go && pos < out_cap) {
-    out_recs[pos] = r;
+  flags[i] = fl;
+}
+
+// rank[i] = exclusive prefix of the flag pairs; slot q spans
+// [slot_begin[q], slot_begin[q+1]) of the active list.
+__global__ __launch_bounds__(256) void k_route_scatter(const uint4 *__restrict__ recs,
+                                                       const uint32_t *__restrict__ rec_slot, uint32_t n,
+                                                       const unsigned long long *__restrict__ flags,
+                                                       const unsigned long long *__restrict__ rank,
+                                                       const uint32_t *__restrict__ slot_begin,
+                                                       const int32_t *__restrict__ slot_next,
+                                                       const uint32_t *__restrict__ next_begin, uint4 *__restrict__ out_recs,
+                                                       uint32_t *__restrict__ out_slot, uint32_t out_cap)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const unsigned long long fl = flags[i];
+  if (!fl) return;
+  const uint32_t slot = rec_slot[i];
+  const uint32_t side = (uint32_t)(fl >> 32) ? 1u : 0u;
+  const int32_t ns = slot_next[2 * slot + side];
+  if (ns < 0) return;
+  const unsigned long long r0 = rank[slot_begin[slot]], ri = rank[i];
+  const uint32_t k = side ? (uint32_t)(ri >> 32) - (uint32_t)(r0 >> 32) : (uint32_t)ri - (uint32_t)r0;
+  const uint32_t pos = next_begin[ns] + k;
+  // out_cap bounds the write even if routing disagreed with the split counts
+  // (the host compares the per-child totals with them and reports it)
+  if (pos < out_cap) {
+    out_recs[pos] = recs[i];
     out_slot[pos] = (uint32_t)ns;
   }
+}
+
+// Per-slot (left, right) totals of the routed samples from the scan.
+__global__ void k_route_counts(const unsigned long long *__restrict__ rank, const uint32_t *__restrict__ slot_begin,
+                               uint32_t n_slots, uint32_t *__restrict__ ccount)
+{
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n_slots) return;
+  const unsigned long long a = rank[slot_begin[q]], b = rank[slot_begin[q + 1]];
+  ccount[2 * q] = (uint32_t)b - (uint32_t)a;
+  ccount[2 * q + 1] = (uint32_t)(b >> 32) - (uint32_t)(a >> 32);
 }
 
 // ---------------------------------------------------------------------------
