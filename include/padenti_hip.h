@@ -172,6 +172,10 @@ int pd_forest_predict_frames_dev(pd_forest *fo, uint32_t n_frames, const void *d
                                  const uint64_t *post_offsets, int32_t *d_leaves, const uint64_t *leaf_offsets,
                                  int use_graph, float *kernel_ms);
 
+/* HBM probe: average time of a plain device-to-device copy kernel moving
+ * `bytes` (read) + `bytes` (write), over `reps` launches. */
+int pd_copy_bench(int device, uint64_t bytes, int reps, float *ms_per_copy);
+
 /* ---- device memory helpers (for callers without another allocator) ---- */
 int pd_dev_alloc(int device, uint64_t bytes, void **out);
 int pd_dev_free(void *p);

@@ -117,6 +117,7 @@ def lib():
         "pd_forest_predict_posterior": (I32, [VP, VP, U32, U32, VP, VP]),
         "pd_forest_predict_frames_dev": (I32, [VP, U32, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, I32,
                                                C.POINTER(C.c_float)]),
+        "pd_copy_bench": (I32, [I32, U64, I32, C.POINTER(C.c_float)]),
         "pd_dev_alloc": (I32, [I32, U64, C.POINTER(VP)]),
         "pd_dev_free": (I32, [VP]),
         "pd_memcpy_h2d": (I32, [VP, VP, U64]),

@@ -245,7 +245,8 @@ struct Trainer final : TrainerBase {
   DevBuf d_pixels, d_imgs, d_recs;
   // level state
   DevBuf d_actA, d_actB, d_slotA, d_slotB;
-  DevBuf d_H, d_items, d_slot_node, d_slot_atomic, d_zero, d_tot, d_best, d_res;
+  DevBuf d_H, d_items, d_slot_node, d_slot_mode, d_slot_hidx, d_zero, d_tot, d_best, d_res;
+  DevBuf d_cand_key, d_cand_left, d_hlist;
   DevBuf d_tleft, d_tfeat, d_tthr, d_rslot_node, d_slot_next, d_cursor, d_ccount;
   DevBuf d_flags, d_rank, d_sbegin, d_scan_tmp;
   uint32_t cu_count = 256;
@@ -340,7 +341,8 @@ struct Trainer final : TrainerBase {
                                  (int)lds));
     hipLaunchKernelGGL((k_hist<F, NTP>), dim3(n_items), dim3(HIST_THREADS), lds, stream, d_pixels.as<I>(),
                        d_imgs.as<ImgDesc>(), act, d_items.as<HistItem>(), item_base, d_slot_node.as<int32_t>(),
-                       d_slot_atomic.as<uint8_t>(), bnd, tree, nF, nT, nC, d_H.as<uint32_t>());
+                       d_slot_mode.as<uint8_t>(), d_slot_hidx.as<int32_t>(), d_tot.as<uint32_t>(), bnd, tree, nF,
+                       nT, nC, d_H.as<uint32_t>(), d_cand_key.as<unsigned long long>(), d_cand_left.as<uint32_t>());
     HIPCHK(hipGetLastError());
   }
 
@@ -360,10 +362,16 @@ struct Trainer final : TrainerBase {
     }
   }
 
-  // Histograms of one slice of slots into d_H (slice-local slot index).
-  // Returns the number of kernel launches.
+  // Histogram launch(es) of the slots [s0, s1) (slice-local index q = slot -
+  // s0).  With want_h every slot gets its H block (block q).  Otherwise a
+  // slot whose samples fit one chunk is searched inside the kernel (fused)
+  // and needs no H; a slot with several chunks adds into H block hslots-index.
+  // tot: the slice's per-class node totals [q][nC].  Returns the number of
+  // kernel launches; hslots receives the slots that have an H block, in
+  // block order.
   uint32_t hist_slice(const std::vector<SlotInfo> &slots, size_t s0, size_t s1, const uint4 *act,
                       const pd_train_params &P, const Bounds<F> &bnd, uint32_t tree, int mode, uint32_t ch,
+                      const std::vector<uint32_t> &tot, bool want_h, std::vector<uint32_t> &hslots,
                       const Event *ev_begin = nullptr, const Event *ev_end = nullptr)
   {
     const uint32_t nF = P.n_features, nT = P.n_thresholds;
@@ -372,33 +380,51 @@ struct Trainer final : TrainerBase {
     const uint32_t ns = (uint32_t)(s1 - s0);
     std::vector<HistItem> items;
     std::vector<uint32_t> item_start(ns + 1, 0);
-    std::vector<int32_t> snode(ns);
-    std::vector<uint8_t> satom(ns);
+    std::vector<int32_t> snode(ns), shidx(ns, -1);
+    std::vector<uint8_t> smode(ns, 0);
     std::vector<uint32_t> zero;
+    hslots.clear();
     for (uint32_t q = 0; q < ns; q++) {
       const SlotInfo &si = slots[s0 + q];
       snode[q] = si.node;
       item_start[q] = (uint32_t)items.size();
       const uint32_t cnt = si.trainable ? si.count : 0u;
-      const uint32_t chunks = (cnt + ch - 1) / ch;
-      satom[q] = chunks == 1 ? 0 : 1;
-      if (chunks != 1) zero.push_back(q);
+      const uint32_t chunks = std::max<uint32_t>(1u, (cnt + ch - 1) / ch);
+      if (want_h) {
+        shidx[q] = (int32_t)q;
+        hslots.push_back(q);
+        if (chunks > 1) {
+          smode[q] = SLOT_ATOMIC;
+          zero.push_back(q);
+        }
+      } else if (chunks == 1) {
+        smode[q] = SLOT_FUSED;
+      } else {
+        smode[q] = SLOT_ATOMIC;
+        shidx[q] = (int32_t)hslots.size();
+        zero.push_back((uint32_t)hslots.size());
+        hslots.push_back(q);
+      }
       for (uint32_t k = 0; k < chunks; k++) {
         const uint32_t b = si.begin + k * ch;
-        const uint32_t c = std::min(ch, cnt - k * ch);
+        const uint32_t c = cnt ? std::min(ch, cnt - k * ch) : 0u;
         for (uint32_t ft = 0; ft < ftiles; ft++) items.push_back(HistItem{q, b, c, ft});
       }
     }
     item_start[ns] = (uint32_t)items.size();
     d_items.ensure(std::max<size_t>(items.size(), 1) * sizeof(HistItem));
     d_slot_node.ensure(ns * sizeof(int32_t));
-    d_slot_atomic.ensure(ns);
-    d_H.ensure(per * ns * sizeof(uint32_t));
-    if (!items.empty())
-      HIPCHK(hipMemcpyAsync(d_items.p, items.data(), items.size() * sizeof(HistItem), hipMemcpyHostToDevice,
-                            stream));
+    d_slot_mode.ensure(ns);
+    d_slot_hidx.ensure(ns * sizeof(int32_t));
+    d_tot.ensure((size_t)ns * nC * sizeof(uint32_t));
+    d_H.ensure(per * std::max<size_t>(hslots.size(), 1) * sizeof(uint32_t));
+    d_cand_key.ensure((size_t)ns * ftiles * sizeof(unsigned long long));
+    d_cand_left.ensure((size_t)ns * ftiles * nC * sizeof(uint32_t));
+    HIPCHK(hipMemcpyAsync(d_items.p, items.data(), items.size() * sizeof(HistItem), hipMemcpyHostToDevice, stream));
     HIPCHK(hipMemcpyAsync(d_slot_node.p, snode.data(), ns * sizeof(int32_t), hipMemcpyHostToDevice, stream));
-    HIPCHK(hipMemcpyAsync(d_slot_atomic.p, satom.data(), ns, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(d_slot_mode.p, smode.data(), ns, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(d_slot_hidx.p, shidx.data(), ns * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(d_tot.p, tot.data(), (size_t)ns * nC * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
     if (!zero.empty()) {
       d_zero.ensure(zero.size() * sizeof(uint32_t));
       HIPCHK(hipMemcpyAsync(d_zero.p, zero.data(), zero.size() * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
@@ -427,44 +453,53 @@ struct Trainer final : TrainerBase {
     return launches;
   }
 
+  // Split search over the H blocks: block i belongs to slot hlist[i].
   template <int NCM>
-  void launch_split_t(uint32_t ns, uint32_t nT, uint32_t nF, int mode, float *all_gains)
+  void launch_split_t(uint32_t nh, const uint32_t *d_list, uint32_t nT, uint32_t nF, int mode, float *all_gains)
   {
     const uint32_t npairs = nT * nF;
     const uint32_t ppb = SPLIT_THREADS * 8;
     const uint32_t bps = (npairs + ppb - 1) / ppb;
     const size_t per = (size_t)nC * npairs;
     if (mode == PD_MODE_LEVEL_BATCHED) {
-      hipLaunchKernelGGL((k_split<NCM>), dim3(ns * bps), dim3(SPLIT_THREADS), 0, stream, d_H.as<uint32_t>(),
-                         d_tot.as<uint32_t>(), nC, nT, nF, bps, ppb, d_best.as<unsigned long long>(), all_gains);
+      hipLaunchKernelGGL((k_split<NCM>), dim3(nh * bps), dim3(SPLIT_THREADS), 0, stream, d_H.as<uint32_t>(),
+                         d_tot.as<uint32_t>(), nC, nT, nF, bps, ppb, d_best.as<unsigned long long>(), all_gains,
+                         d_list);
       HIPCHK(hipGetLastError());
     } else {
-      for (uint32_t q = 0; q < ns; q++) {
-        hipLaunchKernelGGL((k_split<NCM>), dim3(bps), dim3(SPLIT_THREADS), 0, stream,
-                           d_H.as<uint32_t>() + q * per, d_tot.as<uint32_t>() + (size_t)q * nC, nC, nT, nF, bps,
-                           ppb, d_best.as<unsigned long long>() + q,
-                           all_gains ? all_gains + (size_t)q * npairs : nullptr);
+      for (uint32_t i = 0; i < nh; i++) {
+        hipLaunchKernelGGL((k_split<NCM>), dim3(bps), dim3(SPLIT_THREADS), 0, stream, d_H.as<uint32_t>() + i * per,
+                           d_tot.as<uint32_t>(), nC, nT, nF, bps, ppb, d_best.as<unsigned long long>(), all_gains,
+                           d_list + i);
         HIPCHK(hipGetLastError());
       }
     }
   }
 
-  // Split search + finalize of a slice; res_out[q*(nC+2) + {0,1,2..}].
-  void split_slice(uint32_t ns, const std::vector<uint32_t> &tot, uint32_t nT, uint32_t nF, int mode,
-                   std::vector<uint32_t> &res_out)
+  // Split search + finalize of a slice of ns slots whose H blocks are
+  // hslots (the fused slots were searched by the histogram kernel);
+  // res_out[q*(nC+2) + {0: k, 1: gain bits, 2..: left counts}].
+  void split_slice(uint32_t ns, const std::vector<uint32_t> &hslots, uint32_t nT, uint32_t nF, int mode,
+                   std::vector<uint32_t> &res_out, float *all_gains = nullptr)
   {
-    d_tot.ensure((size_t)ns * nC * sizeof(uint32_t));
+    const uint32_t nh = (uint32_t)hslots.size();
     d_best.ensure((size_t)ns * sizeof(uint64_t));
     d_res.ensure((size_t)ns * (nC + 2) * sizeof(uint32_t));
-    HIPCHK(hipMemcpyAsync(d_tot.p, tot.data(), (size_t)ns * nC * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+    d_hlist.ensure(std::max<size_t>(nh, 1) * sizeof(uint32_t));
     HIPCHK(hipMemsetAsync(d_best.p, 0, (size_t)ns * sizeof(uint64_t), stream));
-    if (nC <= 8) launch_split_t<8>(ns, nT, nF, mode, nullptr);
-    else if (nC <= 16) launch_split_t<16>(ns, nT, nF, mode, nullptr);
-    else if (nC <= 32) launch_split_t<32>(ns, nT, nF, mode, nullptr);
-    else launch_split_t<64>(ns, nT, nF, mode, nullptr);
+    if (nh) {
+      HIPCHK(hipMemcpyAsync(d_hlist.p, hslots.data(), nh * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+      const uint32_t *dl = d_hlist.as<uint32_t>();
+      if (nC <= 8) launch_split_t<8>(nh, dl, nT, nF, mode, all_gains);
+      else if (nC <= 16) launch_split_t<16>(nh, dl, nT, nF, mode, all_gains);
+      else if (nC <= 32) launch_split_t<32>(nh, dl, nT, nF, mode, all_gains);
+      else launch_split_t<64>(nh, dl, nT, nF, mode, all_gains);
+    }
     const uint32_t threads = ns * (nC + 2);
     hipLaunchKernelGGL(k_finalize, dim3((threads + 255) / 256), dim3(256), 0, stream, d_H.as<uint32_t>(),
-                       d_best.as<unsigned long long>(), ns, nC, nT, nF, d_res.as<uint32_t>());
+                       d_best.as<unsigned long long>(), d_slot_mode.as<uint8_t>(), d_slot_hidx.as<int32_t>(),
+                       d_cand_key.as<unsigned long long>(), d_cand_left.as<uint32_t>(), ns, nC, nT, nF,
+                       d_res.as<uint32_t>());
     HIPCHK(hipGetLastError());
     res_out.resize((size_t)ns * (nC + 2));
     HIPCHK(hipMemcpyAsync(res_out.data(), d_res.p, res_out.size() * sizeof(uint32_t), hipMemcpyDeviceToHost,
@@ -516,7 +551,7 @@ struct Trainer final : TrainerBase {
     Event e0, e1, e2, e3, e4;
     const size_t per = (size_t)nC * nT * nF;
     const size_t per_bytes = per * sizeof(uint32_t);
-    const size_t slots_per_slice = std::max<size_t>(1, hist_budget / per_bytes);
+    const size_t h_per_slice = std::max<size_t>(1, hist_budget / per_bytes);   // H blocks per slice
 
     for (uint32_t d = 1; d < end_depth; d++) {
       const auto t_start = std::chrono::steady_clock::now();
@@ -534,17 +569,26 @@ struct Trainer final : TrainerBase {
       st.evals = (double)active * nF * nT;
       uint32_t ch = pick_chunk(active, (nF + 63) / 64);
       std::vector<uint8_t> split_flag(slots.size(), 0);
-      for (size_t s0 = 0; s0 < slots.size(); s0 += slots_per_slice) {
-        const size_t s1 = std::min(slots.size(), s0 + slots_per_slice);
+      for (size_t s0 = 0; s0 < slots.size();) {
+        // a slice holds at most h_per_slice multi-chunk slots (the only ones
+        // that need an H block)
+        size_t s1 = s0, nh = 0;
+        while (s1 < slots.size()) {
+          const uint32_t cnt = slots[s1].trainable ? slots[s1].count : 0u;
+          const bool needs_h = cnt > ch;
+          if (needs_h && nh == h_per_slice) break;
+          nh += needs_h ? 1 : 0;
+          s1++;
+        }
         const uint32_t ns = (uint32_t)(s1 - s0);
         st.slices++;
-        st.hist_launches += hist_slice(slots, s0, s1, act, P, bnd, tree_id, mode, ch, &e0, &e1);
         std::vector<uint32_t> tot((size_t)ns * nC);
         for (uint32_t q = 0; q < ns; q++)
           std::copy(&classTot[(size_t)slots[s0 + q].node * nC], &classTot[(size_t)slots[s0 + q].node * nC] + nC,
                     &tot[(size_t)q * nC]);
-        std::vector<uint32_t> res;
-        split_slice(ns, tot, nT, nF, mode, res);
+        std::vector<uint32_t> hslots, res;
+        st.hist_launches += hist_slice(slots, s0, s1, act, P, bnd, tree_id, mode, ch, tot, false, hslots, &e0, &e1);
+        split_slice(ns, hslots, nT, nF, mode, res);
         HIPCHK(hipEventRecord(e2.e, stream));
         HIPCHK(hipEventSynchronize(e2.e));
         st.hist_ms += elapsed_ms(e0, e1);
@@ -585,6 +629,7 @@ struct Trainer final : TrainerBase {
           split_flag[s0 + q] = 1;
           st.split_nodes++;
         }
+        s0 = s1;
       }
       if (d + 1 >= end_depth) {
         st.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
@@ -708,7 +753,9 @@ struct Trainer final : TrainerBase {
       HIPCHK(hipMemcpyAsync(d_actA.p, act.data(), act.size() * sizeof(uint4), hipMemcpyHostToDevice, stream));
     const Bounds<F> bnd = make_bounds<F>(P);
     const uint32_t ch = pick_chunk(act.size(), (P.n_features + 63) / 64);
-    hist_slice(slots, 0, slots.size(), d_actA.as<uint4>(), P, bnd, tree_id, PD_MODE_LEVEL_BATCHED, ch);
+    std::vector<uint32_t> tot((size_t)n_nodes * nC, 0u), hslots;
+    hist_slice(slots, 0, slots.size(), d_actA.as<uint4>(), P, bnd, tree_id, PD_MODE_LEVEL_BATCHED, ch, tot, true,
+               hslots);
     const size_t per_n = (size_t)nC * P.n_thresholds * P.n_features;
     HIPCHK(hipMemcpyAsync(H_out, d_H.p, per_n * n_nodes * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
@@ -1110,28 +1157,31 @@ int pd_split_search(int device, uint32_t n_nodes, uint32_t n_classes, uint32_t n
             "bad split-search shape");
     require(H && tot && best_k, "null split-search buffer");
     set_device(device);
-    // reuse the trainer's split path on a throw-away depth trainer
+    // the trainer's split path on a throw-away trainer: every node has an H block
     Trainer<DepthU16Feature> t(device, n_classes);
     const size_t per = (size_t)n_classes * n_thresholds * n_features;
+    const uint32_t ftiles = (n_features + 63) / 64;
     t.d_H.ensure(per * n_nodes * sizeof(uint32_t));
+    t.d_tot.ensure((size_t)n_nodes * n_classes * sizeof(uint32_t));
+    t.d_slot_mode.ensure(n_nodes);
+    t.d_slot_hidx.ensure(n_nodes * sizeof(int32_t));
+    t.d_cand_key.ensure((size_t)n_nodes * ftiles * sizeof(unsigned long long));
+    t.d_cand_left.ensure((size_t)n_nodes * ftiles * n_classes * sizeof(uint32_t));
+    std::vector<uint8_t> mode(n_nodes, 0);
+    std::vector<int32_t> hidx(n_nodes);
+    std::vector<uint32_t> hslots(n_nodes), res;
+    for (uint32_t q = 0; q < n_nodes; q++) hidx[q] = (int32_t)(hslots[q] = q);
     HIPCHK(hipMemcpy(t.d_H.p, H, per * n_nodes * sizeof(uint32_t), hipMemcpyHostToDevice));
-    std::vector<uint32_t> tv(tot, tot + (size_t)n_nodes * n_classes), res;
+    HIPCHK(hipMemcpy(t.d_tot.p, tot, (size_t)n_nodes * n_classes * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(t.d_slot_mode.p, mode.data(), n_nodes, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(t.d_slot_hidx.p, hidx.data(), n_nodes * sizeof(int32_t), hipMemcpyHostToDevice));
     DevBuf dg;
-    if (all_gains) {
-      dg.ensure((size_t)n_nodes * n_thresholds * n_features * sizeof(float));
-      t.d_tot.ensure((size_t)n_nodes * n_classes * sizeof(uint32_t));
-      t.d_best.ensure((size_t)n_nodes * sizeof(uint64_t));
-      HIPCHK(hipMemcpy(t.d_tot.p, tv.data(), tv.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
-      HIPCHK(hipMemset(t.d_best.p, 0, (size_t)n_nodes * sizeof(uint64_t)));
-      if (n_classes <= 8) t.launch_split_t<8>(n_nodes, n_thresholds, n_features, PD_MODE_LEVEL_BATCHED, dg.as<float>());
-      else if (n_classes <= 16) t.launch_split_t<16>(n_nodes, n_thresholds, n_features, PD_MODE_LEVEL_BATCHED, dg.as<float>());
-      else if (n_classes <= 32) t.launch_split_t<32>(n_nodes, n_thresholds, n_features, PD_MODE_LEVEL_BATCHED, dg.as<float>());
-      else t.launch_split_t<64>(n_nodes, n_thresholds, n_features, PD_MODE_LEVEL_BATCHED, dg.as<float>());
-      HIPCHK(hipStreamSynchronize(t.stream));
+    if (all_gains) dg.ensure((size_t)n_nodes * n_thresholds * n_features * sizeof(float));
+    t.split_slice(n_nodes, hslots, n_thresholds, n_features, PD_MODE_LEVEL_BATCHED, res,
+                  all_gains ? dg.as<float>() : nullptr);
+    if (all_gains)
       HIPCHK(hipMemcpy(all_gains, dg.p, (size_t)n_nodes * n_thresholds * n_features * sizeof(float),
                        hipMemcpyDeviceToHost));
-    }
-    t.split_slice(n_nodes, tv, n_thresholds, n_features, PD_MODE_LEVEL_BATCHED, res);
     for (uint32_t q = 0; q < n_nodes; q++) {
       best_k[q] = res[(size_t)q * (n_classes + 2)];
       if (best_gain) std::memcpy(&best_gain[q], &res[(size_t)q * (n_classes + 2) + 1], 4);
@@ -1273,6 +1323,31 @@ int pd_forest_predict_frames_dev(pd_forest *fo, uint32_t n_frames, const void *d
     require(fo != nullptr, "null forest");
     fo->impl->predict_frames(n_frames, d_pixels, img_offsets, widths, heights, d_masks, mask_offsets, d_post,
                              post_offsets, d_leaves, leaf_offsets, use_graph, kernel_ms);
+  });
+}
+
+int pd_copy_bench(int device, uint64_t bytes, int reps, float *ms_per_copy)
+{
+  return guarded([&] {
+    require(bytes >= 4096 && reps >= 1 && ms_per_copy, "bad copy-bench arguments");
+    set_device(device);
+    DevBuf a, b;
+    a.ensure(bytes);
+    b.ensure(bytes);
+    HIPCHK(hipMemset(a.p, 1, bytes));
+    const size_t n = bytes / 16;
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    const uint32_t blocks = (uint32_t)prop.multiProcessorCount * 8;
+    hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, nullptr, a.as<uint4>(), b.as<uint4>(), n);
+    HIPCHK(hipDeviceSynchronize());
+    Event e0, e1;
+    HIPCHK(hipEventRecord(e0.e, nullptr));
+    for (int r = 0; r < reps; r++)
+      hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, nullptr, a.as<uint4>(), b.as<uint4>(), n);
+    HIPCHK(hipEventRecord(e1.e, nullptr));
+    HIPCHK(hipEventSynchronize(e1.e));
+    *ms_per_copy = (float)(elapsed_ms(e0, e1) / reps);
   });
 }
 

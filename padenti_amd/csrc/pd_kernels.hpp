@@ -76,7 +76,17 @@ constexpr size_t hist_lds_bytes(uint32_t nC)
          (size_t)NTP * 64;                                                   // ranks L_t
 }
 
-// Fill the Prep word of every sample record.
+__device__ __forceinline__ uint64_t wave_max_u64(uint64_t v)
+{
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    uint64_t w = __shfl_xor(v, o, 64);
+    v = w > v ? w : v;
+  }
+  return v;
+}
+
+// Fill the Prep words of every sample record.
 template <class F>
 __global__ void k_prepare(const typename F::ImgType *__restrict__ pixels, const ImgDesc *__restrict__ imgs,
                           uint4 *__restrict__ recs, uint32_t n)
@@ -120,12 +130,26 @@ __device__ __forceinline__ void hist_sample(const typename F::ImgType *__restric
   if (fv) atomicAdd(&bins[(rec_cls(r) * RW + (rk >> 1)) * 64u + lane], 1u << ((rk & 1u) * 16u));
 }
 
+// Per-slot mode bits of the histogram kernel.
+constexpr uint8_t SLOT_ATOMIC = 1;   // several chunks add into H (else plain stores)
+constexpr uint8_t SLOT_FUSED = 2;    // one chunk: evaluate the gains in-kernel, no H
+
+// Gain of pair (t, f) read from the in-LDS prefix counts of this lane's
+// candidate: left(c) = prefix_c[L_t].  Op order of learn_best_feature.cl:63-93
+// (h: the node entropy, N: the node total, both computed per thread).
+__device__ __forceinline__ uint32_t prefix_at(const uint32_t *bins, int RW, uint32_t c, uint32_t l, uint32_t lane)
+{
+  return (bins[(c * RW + (l >> 1)) * 64u + lane] >> ((l & 1u) * 16u)) & 0xffffu;
+}
+
 template <class F, int NTP>
 __global__ __launch_bounds__(HIST_THREADS) void k_hist(
     const typename F::ImgType *__restrict__ pixels, const ImgDesc *__restrict__ imgs,
     const uint4 *__restrict__ recs, const HistItem *__restrict__ items, uint32_t item_base,
-    const int32_t *__restrict__ slot_node, const uint8_t *__restrict__ slot_atomic, Bounds<F> bnd,
-    uint32_t tree, uint32_t nF, uint32_t nT, uint32_t nC, uint32_t *__restrict__ H)
+    const int32_t *__restrict__ slot_node, const uint8_t *__restrict__ slot_mode,
+    const int32_t *__restrict__ slot_hidx, const uint32_t *__restrict__ tot, Bounds<F> bnd, uint32_t tree,
+    uint32_t nF, uint32_t nT, uint32_t nC, uint32_t *__restrict__ H, unsigned long long *__restrict__ cand_key,
+    uint32_t *__restrict__ cand_left)
 {
   using T = typename F::FeatType;
   constexpr int FD = F::FeatDim;
@@ -208,9 +232,9 @@ __global__ __launch_bounds__(HIST_THREADS) void k_hist(
 
   // Flush: per class (classes strided over waves) turn this lane's rank bins
   // into prefix counts in place, then count(c, t) = prefix[L_t].
-  const bool atom = slot_atomic[it.slot] != 0;
+  const uint8_t mode = slot_mode[it.slot];
+  const int32_t hidx = slot_hidx[it.slot];
   const size_t per_class = (size_t)nT * nF;
-  uint32_t *Hs = H + (size_t)it.slot * nC * per_class;
   for (uint32_t c = wave; c < nC; c += NW) {
     uint32_t *b = bins + (size_t)c * RW * 64u + lane;
     uint32_t run = 0;
@@ -222,19 +246,69 @@ __global__ __launch_bounds__(HIST_THREADS) void k_hist(
       run = hi;
       b[j * 64] = lo | (hi << 16);
     }
-    if (!fv) continue;
-    uint32_t *out = Hs + (size_t)c * per_class + f;
+    if (!fv || hidx < 0) continue;
+    uint32_t *out = H + ((size_t)hidx * nC + c) * per_class + f;
 #pragma unroll
     for (int t = 0; t < NTP; t++) {
       if ((uint32_t)t >= nT) break;
-      const uint32_t l = sl[t * 64 + lane];
-      const uint32_t val = (b[(l >> 1) * 64u] >> ((l & 1u) * 16u)) & 0xffffu;
-      if (atom) {
+      const uint32_t val = prefix_at(bins, RW, c, sl[t * 64 + lane], lane);
+      if (mode & SLOT_ATOMIC) {
         if (val) atomicAdd(out + (size_t)t * nF, val);
       } else {
         out[(size_t)t * nF] = val;
       }
     }
+  }
+  if (!(mode & SLOT_FUSED)) return;
+
+  // Fused split search of this tile: the first maximum over k = t*nF + f of
+  // the FP32 gain (keys: larger gain, then smaller k), then the winner's
+  // left class counts.  Candidates of the other tiles of the node are merged
+  // by the finalize kernel.
+  __shared__ unsigned long long red[NW];
+  __syncthreads();
+  const uint32_t *tt = tot + (size_t)it.slot * nC;
+  int N = 0;
+  for (uint32_t c = 0; c < nC; c++) N += (int)tt[c];
+  const float fN = (float)N;
+  float h = 0.0f;
+  for (uint32_t c = 0; c < nC; c++) h -= plogp((float)tt[c] / fN);
+  unsigned long long best = 0ull;
+  if (fv) {
+    for (uint32_t t = wave; t < nT; t += NW) {
+      const uint32_t l = sl[t * 64 + lane];
+      int nL = 0;
+      for (uint32_t c = 0; c < nC; c++) nL += (int)prefix_at(bins, RW, c, l, lane);
+      const int nR = N - nL;
+      const float fL = (float)nL, fR = (float)nR;
+      float hL = 0.0f, hR = 0.0f;
+      for (uint32_t c = 0; c < nC; c++) {
+        const uint32_t lc = prefix_at(bins, RW, c, l, lane);
+        float tmp = nL ? (float)lc / fL : 0.0f;
+        hL -= plogp(tmp);
+        tmp = nR ? (float)(tt[c] - lc) / fR : 0.0f;
+        hR -= plogp(tmp);
+      }
+      const float a = fL / fN * hL;
+      const float b = fR / fN * hR;
+      const unsigned long long key = split_key(h - (a + b), t * nF + f);
+      best = key > best ? key : best;
+    }
+  }
+  best = wave_max_u64(best);
+  if (lane == 0) red[wave] = best;
+  __syncthreads();
+  unsigned long long m = red[0];
+#pragma unroll
+  for (uint32_t w = 1; w < NW; w++) m = red[w] > m ? red[w] : m;
+  const uint32_t ftiles = (nF + 63) / 64;
+  const size_t ci = (size_t)it.slot * ftiles + it.ftile;
+  if (tid == 0) cand_key[ci] = m;
+  if (m != 0ull && tid < nC) {
+    const uint32_t k = key_index(m);
+    const uint32_t wl = (k % nF) - it.ftile * 64u;
+    const uint32_t l = sl[(k / nF) * 64 + wl];
+    cand_left[ci * nC + tid] = prefix_at(bins, RW, tid, l, wl);
   }
 }
 
@@ -301,26 +375,21 @@ __device__ __forceinline__ float pair_gain(const uint32_t *left, const uint32_t 
   return h - (a + b);
 }
 
-__device__ __forceinline__ uint64_t wave_max_u64(uint64_t v)
-{
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) {
-    uint64_t w = __shfl_xor(v, o, 64);
-    v = w > v ? w : v;
-  }
-  return v;
-}
 
+// slot_list (may be NULL): the slots to search, in H order (slot_list[i] is
+// the slot of H block i); NULL searches slots 0.. with H block = slot.
 template <int NCM>
 __global__ __launch_bounds__(SPLIT_THREADS) void k_split(const uint32_t *__restrict__ H,
                                                          const uint32_t *__restrict__ tot, uint32_t nC,
                                                          uint32_t nT, uint32_t nF, uint32_t blocks_per_slot,
                                                          uint32_t pairs_per_block,
                                                          unsigned long long *__restrict__ best,
-                                                         float *__restrict__ all_gains)
+                                                         float *__restrict__ all_gains,
+                                                         const uint32_t *__restrict__ slot_list)
 {
   __shared__ uint64_t red[SPLIT_THREADS / 64];
-  const uint32_t slot = blockIdx.x / blocks_per_slot;
+  const uint32_t hblk = blockIdx.x / blocks_per_slot;
+  const uint32_t slot = slot_list ? slot_list[hblk] : hblk;
   const uint32_t part = blockIdx.x % blocks_per_slot;
   const uint32_t npairs = nT * nF;
   uint32_t tv[NCM];
@@ -328,7 +397,7 @@ __global__ __launch_bounds__(SPLIT_THREADS) void k_split(const uint32_t *__restr
   for (int l = 0; l < NCM; l++) tv[l] = ((uint32_t)l < nC) ? tot[(size_t)slot * nC + l] : 0u;
   int N;
   const float h = node_entropy<NCM>(tv, nC, N);
-  const uint32_t *Hs = H + (size_t)slot * nC * npairs;
+  const uint32_t *Hs = H + (size_t)hblk * nC * npairs;
   const uint32_t k0 = part * pairs_per_block;
   const uint32_t k1 = min(npairs, k0 + pairs_per_block);
   uint64_t key = 0;
@@ -352,14 +421,34 @@ __global__ __launch_bounds__(SPLIT_THREADS) void k_split(const uint32_t *__restr
   }
 }
 
-// Winner of each slot: res[slot] = {k, gain bits, left[0..nC)}.
+// Winner of each slot: res[slot] = {k, gain bits, left[0..nC)}.  A fused
+// slot takes the largest key over its tiles' candidates (larger gain, then
+// smaller k) and that tile's left counts; any other slot its searched key
+// and the counts of H block slot_hidx[slot].
 __global__ void k_finalize(const uint32_t *__restrict__ H, const unsigned long long *__restrict__ best,
+                           const uint8_t *__restrict__ slot_mode, const int32_t *__restrict__ slot_hidx,
+                           const unsigned long long *__restrict__ cand_key, const uint32_t *__restrict__ cand_left,
                            uint32_t n_slots, uint32_t nC, uint32_t nT, uint32_t nF, uint32_t *__restrict__ res)
 {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t slot = i / (nC + 2), c = i % (nC + 2);
   if (slot >= n_slots) return;
-  const uint64_t key = best[slot];
+  uint64_t key;
+  size_t ci = 0;
+  const bool fused = (slot_mode[slot] & SLOT_FUSED) != 0;
+  if (fused) {
+    const uint32_t ftiles = (nF + 63) / 64;
+    key = 0;
+    for (uint32_t ft = 0; ft < ftiles; ft++) {
+      const uint64_t kk = cand_key[(size_t)slot * ftiles + ft];
+      if (kk > key) {
+        key = kk;
+        ci = (size_t)slot * ftiles + ft;
+      }
+    }
+  } else {
+    key = best[slot];
+  }
   uint32_t k = key_index(key);
   if (k >= nT * nF) k = 0;   // unreachable: every slot evaluates pair 0
   uint32_t *r = res + (size_t)slot * (nC + 2);
@@ -368,8 +457,10 @@ __global__ void k_finalize(const uint32_t *__restrict__ H, const unsigned long l
   } else if (c == 1) {
     const float g = key_gain(key);
     r[1] = __float_as_uint(g);
+  } else if (fused) {
+    r[c] = cand_left[ci * nC + (c - 2)];
   } else {
-    r[c] = H[((size_t)slot * nC + (c - 2)) * nT * nF + k];
+    r[c] = H[((size_t)slot_hidx[slot] * nC + (c - 2)) * nT * nF + k];
   }
 }
 
@@ -572,6 +663,14 @@ __global__ void k_div(const float *__restrict__ p, const float *__restrict__ d, 
   if (i >= n) return;
   const DivPrep dp = div_prep(d[i]);
   q[i] = div_by(p[i], dp, div_fast_ok(dp));
+}
+
+// Plain streaming copy (16 B per lane per step): HBM bandwidth probe and the
+// reference point that scales the FETCH_SIZE / WRITE_SIZE counters.
+__global__ __launch_bounds__(256) void k_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n)
+{
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
 }
 
 __global__ void k_log2_det(const float *__restrict__ x, float *__restrict__ y, uint32_t n)
