@@ -215,17 +215,24 @@ def test_train_c1_bit_exact(gpu, mode):
     tr.close()
 
 
-def test_train_c2_shape_bit_exact(gpu):
+@pytest.mark.parametrize("chunk", [0, 64])
+def test_train_c2_shape_bit_exact(gpu, chunk):
     """C2 shapes (320x240 uint16 capsules, 8 classes, 2000 x 20) on 12 images,
-    depth 10; a small histogram budget forces several slices per level."""
+    depth 10.  chunk 0: automatic work items (nodes below one chunk are
+    searched inside the histogram kernel); chunk 64: every node above 64
+    samples is split over several work items adding into H, and a small
+    histogram budget forces several slices per level."""
     ts = synth.training_set("c2", n_images=12, samples=400)
     p = synth.params_for("c2")
     tree = pa.Tree(1, 10, pa.PD_FEAT_DEPTH_U16, 8)
     tr = pa.HIPTreeTrainer(pa.PD_FEAT_DEPTH_U16, 8)
     tr.set_training_set(ts)
-    tr.set_hist_budget(3 * 8 * 20 * 2000 * 4)
+    if chunk:
+        tr.set_chunk(chunk)
+        tr.set_hist_budget(3 * 8 * 20 * 2000 * 4)
     diag = tr.train(tree, ts, p, 1, 10, diagnostics=True)
-    assert max(s["slices"] for s in tr.last_level_stats) > 1
+    if chunk:
+        assert max(s["slices"] for s in tr.last_level_stats) > 1
     out = orc.train_tree(orc.Tset(pa.PD_FEAT_DEPTH_U16, ts), p, ts.priors, 1, 10)
     _check_tree_equal(tree, out, diag)
     tr.close()
