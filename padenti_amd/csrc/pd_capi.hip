@@ -250,6 +250,7 @@ struct Trainer final : TrainerBase {
   DevBuf d_tleft, d_tfeat, d_tthr, d_rslot_node, d_slot_next, d_cursor, d_ccount;
   DevBuf d_flags, d_rank, d_sbegin, d_scan_tmp;
   uint32_t cu_count = 256;
+  uint32_t n_xcc = 1;
   size_t lds_limit = 65536;
 
   Trainer(int dev, uint32_t ncl)
@@ -262,6 +263,34 @@ struct Trainer final : TrainerBase {
     HIPCHK(hipGetDeviceProperties(&prop, device));
     cu_count = (uint32_t)prop.multiProcessorCount;
     lds_limit = prop.sharedMemPerBlock;
+    int xcc = 1;
+    if (hipDeviceGetAttribute(&xcc, hipDeviceAttributeNumberOfXccs, device) != hipSuccess || xcc < 1) {
+      (void)hipGetLastError();
+      xcc = 1;
+    }
+    n_xcc = (uint32_t)xcc;
+  }
+
+  // XCD-aware launch order.  Workgroup b of a launch runs on XCD b % n_xcc,
+  // and each XCD has its own L2.  The items of one (slot, chunk) group (its
+  // candidate tiles, which walk the same samples and so the same images)
+  // are all placed on one XCD: group g -> XCD g % n_xcc, and the position
+  // within that XCD's sequence is b / n_xcc.  Shorter sequences are padded
+  // with NULL_SLOT items.  `group` gives the group id of every item (items
+  // of a group are contiguous).
+  void xcd_order(std::vector<HistItem> &items, const std::vector<uint32_t> &group_start) const
+  {
+    const uint32_t X = n_xcc;
+    if (X <= 1 || items.empty()) return;
+    std::vector<std::vector<HistItem>> lanes(X);
+    const size_t ng = group_start.size() - 1;
+    for (size_t g = 0; g < ng; g++)
+      for (uint32_t i = group_start[g]; i < group_start[g + 1]; i++) lanes[g % X].push_back(items[i]);
+    size_t len = 0;
+    for (auto &l : lanes) len = std::max(len, l.size());
+    items.assign(len * X, HistItem{NULL_SLOT, 0u, 0u, 0u});
+    for (uint32_t x = 0; x < X; x++)
+      for (size_t j = 0; j < lanes[x].size(); j++) items[j * X + x] = lanes[x][j];
   }
 
   void set_data(uint32_t ni, const void *pixels, const uint64_t *img_off, const uint32_t *widths,
@@ -382,7 +411,7 @@ struct Trainer final : TrainerBase {
     std::vector<uint32_t> item_start(ns + 1, 0);
     std::vector<int32_t> snode(ns), shidx(ns, -1);
     std::vector<uint8_t> smode(ns, 0);
-    std::vector<uint32_t> zero;
+    std::vector<uint32_t> zero, group_start;
     hslots.clear();
     for (uint32_t q = 0; q < ns; q++) {
       const SlotInfo &si = slots[s0 + q];
@@ -408,10 +437,13 @@ struct Trainer final : TrainerBase {
       for (uint32_t k = 0; k < chunks; k++) {
         const uint32_t b = si.begin + k * ch;
         const uint32_t c = cnt ? std::min(ch, cnt - k * ch) : 0u;
+        group_start.push_back((uint32_t)items.size());
         for (uint32_t ft = 0; ft < ftiles; ft++) items.push_back(HistItem{q, b, c, ft});
       }
     }
     item_start[ns] = (uint32_t)items.size();
+    group_start.push_back((uint32_t)items.size());
+    if (mode == PD_MODE_LEVEL_BATCHED) xcd_order(items, group_start);
     d_items.ensure(std::max<size_t>(items.size(), 1) * sizeof(HistItem));
     d_slot_node.ensure(ns * sizeof(int32_t));
     d_slot_mode.ensure(ns);

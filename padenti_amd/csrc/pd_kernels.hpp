@@ -37,8 +37,10 @@ struct ImgDesc {
   uint32_t w, h;
 };
 
+constexpr uint32_t NULL_SLOT = 0xffffffffu;   // padding item: the block exits at once
+
 struct HistItem {
-  uint32_t slot;    // slice-local frontier slot
+  uint32_t slot;    // slice-local frontier slot (NULL_SLOT: padding)
   uint32_t begin;   // first entry in the active sample list
   uint32_t count;   // number of samples (<= 65535)
   uint32_t ftile;   // candidate tile: features [64*ftile, 64*ftile+64)
@@ -164,6 +166,7 @@ __global__ __launch_bounds__(HIST_THREADS) void k_hist(
   uint8_t *sl = (uint8_t *)(st + NTP * 64);
 
   const HistItem it = items[item_base + blockIdx.x];
+  if (it.slot == NULL_SLOT) return;   // whole block: no barrier reached
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = tid & 63u;
   const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
