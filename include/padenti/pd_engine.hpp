@@ -1,0 +1,1105 @@
+// Engine of the gfx950 random-forest hot path: trainer (level loop,
+// slicing, node update, routing) and forest predictor, templated on the
+// feature functor.  Included by the library's C ABI (pd_capi.hip) for the
+// built-in hooks and by a user's hipcc-compiled translation unit for a
+// custom hook (pd_plugin.hpp).
+//
+// Reference: src/padenti/cl_tree_trainer_impl.hpp:203-255 (level loop),
+// cl_tree_trainer_impl_init.hpp:24-365 (init, frontier),
+// cl_tree_trainer_impl_traverse_ts.hpp:59-704 (traversal + accumulation),
+// cl_tree_trainer_impl_learn_best_featthr.hpp:25-283 (split + node update),
+// cl_classifier_impl.hpp:41-557 (classifier).
+#pragma once
+#include "../padenti_hip.h"
+#include "pd_kernels.hpp"
+
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#pragma clang fp contract(off)
+
+namespace pd {
+namespace engine {
+
+inline thread_local std::string g_err;
+
+struct ArgError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct HipError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct NoMem : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct StateError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct ParityError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+#define HIPCHK(x)                                                                              \
+  do {                                                                                         \
+    hipError_t e_ = (x);                                                                       \
+    if (e_ != hipSuccess) throw HipError(std::string(#x) + ": " + hipGetErrorString(e_));      \
+  } while (0)
+
+template <class Fn>
+inline int guarded(Fn &&fn)
+{
+  try {
+    fn();
+    return PD_OK;
+  } catch (const ArgError &e) {
+    g_err = e.what();
+    return PD_ERR_ARG;
+  } catch (const NoMem &e) {
+    g_err = e.what();
+    return PD_ERR_NOMEM;
+  } catch (const HipError &e) {
+    g_err = e.what();
+    return PD_ERR_HIP;
+  } catch (const StateError &e) {
+    g_err = e.what();
+    return PD_ERR_STATE;
+  } catch (const ParityError &e) {
+    g_err = e.what();
+    return PD_ERR_PARITY;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return PD_ERR_STATE;
+  }
+}
+
+inline void require(bool ok, const std::string &msg)
+{
+  if (!ok) throw ArgError(msg);
+}
+
+inline void set_device(int device)
+{
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) throw HipError("no HIP device available");
+  require(device >= 0 && device < n, "device index out of range");
+  HIPCHK(hipSetDevice(device));
+}
+
+// Growable device buffer.
+struct DevBuf {
+  void *p = nullptr;
+  size_t cap = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { release(); }
+  void release()
+  {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+  void ensure(size_t bytes)
+  {
+    if (bytes <= cap && p) return;
+    release();
+    if (bytes == 0) bytes = 16;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      p = nullptr;
+      throw NoMem("hipMalloc of " + std::to_string(bytes) + " bytes failed");
+    }
+    cap = bytes;
+  }
+  template <class T>
+  T *as() const
+  {
+    return (T *)p;
+  }
+};
+
+struct Event {
+  hipEvent_t e = nullptr;
+  Event() { HIPCHK(hipEventCreate(&e)); }
+  ~Event()
+  {
+    if (e) (void)hipEventDestroy(e);
+  }
+};
+
+inline double elapsed_ms(const Event &a, const Event &b)
+{
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, a.e, b.e));
+  return (double)ms;
+}
+
+// FP64 gain of a split (the FP64 host oracle of the chosen split's gain).
+inline double gain_f64(const uint32_t *left, const uint32_t *tot, uint32_t nC)
+{
+  double nL = 0, nR = 0;
+  for (uint32_t l = 0; l < nC; l++) {
+    nL += left[l];
+    nR += (double)(tot[l] - left[l]);
+  }
+  const double N = nL + nR;
+  double h = 0, hL = 0, hR = 0;
+  for (uint32_t l = 0; l < nC; l++) {
+    double p = tot[l] / N;
+    if (p > 1e-6) h -= p * std::log2(p);
+    p = nL > 0 ? left[l] / nL : 0.0;
+    if (p > 1e-6) hL -= p * std::log2(p);
+    p = nR > 0 ? (tot[l] - left[l]) / nR : 0.0;
+    if (p > 1e-6) hR -= p * std::log2(p);
+  }
+  return h - (nL / N * hL + nR / N * hR);
+}
+
+// compiled threshold-count buckets of the histogram kernel (exact for the
+// common 10 / 20 / 32; a larger bucket pads with thresholds no finite
+// response exceeds)
+inline int n_thr_bucket(uint32_t nT)
+{
+  static const int buckets[] = {8, 10, 16, 20, 24, 32, 48, 64};
+  for (int b : buckets)
+    if (nT <= (uint32_t)b) return b;
+  return 64;
+}
+
+template <class F>
+Bounds<F> make_bounds(const pd_train_params &P)
+{
+  using T = typename F::FeatType;
+  Bounds<F> b;
+  for (int j = 0; j < F::FeatDim; j++) {
+    b.lo[j] = (T)P.feat_low[j];
+    b.hi[j] = (T)P.feat_up[j];
+  }
+  b.tlo = (T)P.thr_low;
+  b.thi = (T)P.thr_up;
+  return b;
+}
+
+inline void check_params(const pd_train_params &P, int fd, bool is_short)
+{
+  require(P.n_features >= 1, "n_features must be >= 1");
+  require(P.n_thresholds >= 1 && P.n_thresholds <= 64, "n_thresholds must be in [1, 64]");
+  if (is_short) {
+    for (int j = 0; j < fd; j++)
+      require(P.feat_low[j] >= -32768 && P.feat_up[j] <= 32767 && P.feat_up[j] - P.feat_low[j] <= 32767,
+              "short feature bounds out of range");
+    require(P.thr_low >= -32768 && P.thr_up <= 32767 && P.thr_up - P.thr_low <= 32767,
+            "short threshold bounds out of range");
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Trainer
+// ---------------------------------------------------------------------------
+struct TrainerBase {
+  int device = 0;
+  uint32_t nC = 0;
+  uint64_t hist_budget = 8ull << 30;
+  uint32_t chunk = 0;
+  hipStream_t stream = nullptr;
+  virtual ~TrainerBase()
+  {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  virtual void set_data(uint32_t n_images, const void *pixels, const uint64_t *img_off, const uint32_t *widths,
+                        const uint32_t *heights, const uint8_t *labels, const uint64_t *lab_off,
+                        const uint32_t *samples, const uint64_t *samp_off, const uint32_t *n_samples) = 0;
+  virtual void train(const pd_train_params &P, uint32_t tree_id, uint32_t depth, uint32_t end_depth, int mode,
+                     const float *prior, int32_t *left, void *feats, void *thrs, float *post, uint32_t *node_k,
+                     float *node_gain, double *node_gain64, pd_level_stats *ls) = 0;
+  virtual void hist_nodes(const pd_train_params &P, uint32_t tree_id, uint32_t n_nodes, const int32_t *nodes,
+                          const int32_t *sample_node, uint32_t *H_out) = 0;
+};
+
+struct SlotInfo {
+  int32_t node;
+  uint32_t begin, count;
+  bool trainable;  // count > perLeafSamplesThr (the consumer gate)
+};
+
+template <class F>
+struct Trainer final : TrainerBase {
+  using T = typename F::FeatType;
+  using I = typename F::ImgType;
+  static constexpr int FD = F::FeatDim;
+
+  // training set
+  uint32_t n_images = 0;
+  uint64_t S = 0;
+  std::vector<uint4> h_recs;
+  std::vector<uint32_t> root_tot;
+  DevBuf d_pixels, d_imgs, d_recs;
+  // level state
+  DevBuf d_actA, d_actB, d_slotA, d_slotB;
+  DevBuf d_H, d_items, d_slot_node, d_slot_mode, d_slot_hidx, d_zero, d_tot, d_best, d_res;
+  DevBuf d_cand_key, d_cand_left, d_hlist;
+  DevBuf d_tleft, d_tfeat, d_tthr, d_rslot_node, d_slot_next, d_cursor, d_ccount;
+  DevBuf d_flags, d_rank, d_sbegin, d_scan_tmp;
+  uint32_t cu_count = 256;
+  uint32_t n_xcc = 1;
+  size_t lds_limit = 65536;
+
+  Trainer(int dev, uint32_t ncl)
+  {
+    device = dev;
+    nC = ncl;
+    set_device(device);
+    HIPCHK(hipStreamCreate(&stream));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    cu_count = (uint32_t)prop.multiProcessorCount;
+    lds_limit = prop.sharedMemPerBlock;
+    int xcc = 1;
+    if (hipDeviceGetAttribute(&xcc, hipDeviceAttributeNumberOfXccs, device) != hipSuccess || xcc < 1) {
+      (void)hipGetLastError();
+      xcc = 1;
+    }
+    n_xcc = (uint32_t)xcc;
+  }
+
+  // XCD-aware launch order.  Workgroup b of a launch runs on XCD b % n_xcc,
+  // and each XCD has its own L2.  The items of one (slot, chunk) group (its
+  // candidate tiles, which walk the same samples and so the same images)
+  // are all placed on one XCD: group g -> XCD g % n_xcc, and the position
+  // within that XCD's sequence is b / n_xcc.  Shorter sequences are padded
+  // with NULL_SLOT items.  `group` gives the group id of every item (items
+  // of a group are contiguous).
+  void xcd_order(std::vector<HistItem> &items, const std::vector<uint32_t> &group_start) const
+  {
+    const uint32_t X = n_xcc;
+    if (X <= 1 || items.empty()) return;
+    std::vector<std::vector<HistItem>> lanes(X);
+    const size_t ng = group_start.size() - 1;
+    for (size_t g = 0; g < ng; g++)
+      for (uint32_t i = group_start[g]; i < group_start[g + 1]; i++) lanes[g % X].push_back(items[i]);
+    size_t len = 0;
+    for (auto &l : lanes) len = std::max(len, l.size());
+    items.assign(len * X, HistItem{NULL_SLOT, 0u, 0u, 0u});
+    for (uint32_t x = 0; x < X; x++)
+      for (size_t j = 0; j < lanes[x].size(); j++) items[j * X + x] = lanes[x][j];
+  }
+
+  void set_data(uint32_t ni, const void *pixels, const uint64_t *img_off, const uint32_t *widths,
+                const uint32_t *heights, const uint8_t *labels, const uint64_t *lab_off, const uint32_t *samples,
+                const uint64_t *samp_off, const uint32_t *n_samples) override
+  {
+    require(ni >= 1 && ni < (1u << 24), "n_images must be in [1, 2^24)");
+    require(pixels && img_off && widths && heights && labels && lab_off && samples && samp_off && n_samples,
+            "null pointer in set_data");
+    set_device(device);
+    std::vector<ImgDesc> imgs(ni);
+    uint64_t total_px = 0;
+    for (uint32_t i = 0; i < ni; i++) {
+      require(widths[i] >= 1 && widths[i] < 65536 && heights[i] >= 1 && heights[i] < 65536,
+              "image size must be in [1, 65535]");
+      imgs[i].off = img_off[i];
+      imgs[i].w = widths[i];
+      imgs[i].h = heights[i];
+      total_px = std::max(total_px, img_off[i] + (uint64_t)widths[i] * heights[i] * F::nChannels);
+    }
+    uint64_t s_total = 0;
+    for (uint32_t i = 0; i < ni; i++) s_total += n_samples[i];
+    require(s_total < (1ull << 32), "too many samples");
+    h_recs.resize(s_total);
+    root_tot.assign(nC, 0u);
+    uint64_t g = 0;
+    for (uint32_t i = 0; i < ni; i++) {
+      const uint8_t *lab = labels + lab_off[i];
+      const uint32_t *smp = samples + samp_off[i];
+      const uint32_t w = widths[i], npx = widths[i] * heights[i];
+      for (uint32_t s = 0; s < n_samples[i]; s++, g++) {
+        const uint32_t px = smp[s];
+        require(px < npx, "sample index outside its image");
+        const uint32_t l = lab[px];
+        require(l >= 1 && l <= nC, "sampled pixel is unlabelled or its label exceeds n_classes");
+        const uint32_t c = l - 1u;
+        root_tot[c]++;  // perClassTotSamples at depth 1, _traverse_ts.hpp:526
+        h_recs[g] = make_uint4(i | (c << 24), (px % w) | ((px / w) << 16), 0u, 0u);
+      }
+    }
+    d_pixels.ensure(total_px * sizeof(I));
+    HIPCHK(hipMemcpy(d_pixels.p, pixels, total_px * sizeof(I), hipMemcpyHostToDevice));
+    d_imgs.ensure(ni * sizeof(ImgDesc));
+    HIPCHK(hipMemcpy(d_imgs.p, imgs.data(), ni * sizeof(ImgDesc), hipMemcpyHostToDevice));
+    d_recs.ensure(std::max<uint64_t>(s_total, 1) * sizeof(uint4));
+    if (s_total) {
+      HIPCHK(hipMemcpy(d_recs.p, h_recs.data(), s_total * sizeof(uint4), hipMemcpyHostToDevice));
+      // per-sample prepared state (the functor's shared per-pixel part)
+      hipLaunchKernelGGL((k_prepare<F>), dim3((uint32_t)((s_total + 255) / 256)), dim3(256), 0, stream,
+                         d_pixels.as<I>(), d_imgs.as<ImgDesc>(), d_recs.as<uint4>(), (uint32_t)s_total);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipStreamSynchronize(stream));
+      HIPCHK(hipMemcpy(h_recs.data(), d_recs.p, s_total * sizeof(uint4), hipMemcpyDeviceToHost));
+    }
+    n_images = ni;
+    S = s_total;
+  }
+
+  uint32_t pick_chunk(uint64_t level_samples, uint32_t ftiles) const
+  {
+    if (chunk) return std::max<uint32_t>(64u, std::min<uint32_t>(chunk, 32768u));
+    const uint64_t target_blocks = (uint64_t)cu_count * 16u;
+    uint64_t c = (level_samples * ftiles + target_blocks - 1) / target_blocks;
+    c = std::max<uint64_t>(c, 1024u);
+    c = std::min<uint64_t>(c, 32768u);
+    return (uint32_t)c;
+  }
+
+  template <int NTP>
+  void launch_hist_t(uint32_t n_items, uint32_t item_base, const uint4 *act, const Bounds<F> &bnd, uint32_t tree,
+                     uint32_t nF, uint32_t nT)
+  {
+    const size_t lds = hist_lds_bytes<F, NTP>(nC);
+    if (lds > lds_limit) throw ArgError("histogram tile needs " + std::to_string(lds) + " B of LDS");
+    if (lds > 65536)
+      HIPCHK(hipFuncSetAttribute((const void *)k_hist<F, NTP>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)lds));
+    hipLaunchKernelGGL((k_hist<F, NTP>), dim3(n_items), dim3(HIST_THREADS), lds, stream, d_pixels.as<I>(),
+                       d_imgs.as<ImgDesc>(), act, d_items.as<HistItem>(), item_base, d_slot_node.as<int32_t>(),
+                       d_slot_mode.as<uint8_t>(), d_slot_hidx.as<int32_t>(), d_tot.as<uint32_t>(), bnd, tree, nF,
+                       nT, nC, d_H.as<uint32_t>(), d_cand_key.as<unsigned long long>(), d_cand_left.as<uint32_t>());
+    HIPCHK(hipGetLastError());
+  }
+
+  void launch_hist(uint32_t n_items, uint32_t item_base, const uint4 *act, const Bounds<F> &bnd, uint32_t tree,
+                   uint32_t nF, uint32_t nT)
+  {
+    if (!n_items) return;
+    switch (n_thr_bucket(nT)) {
+      case 8: launch_hist_t<8>(n_items, item_base, act, bnd, tree, nF, nT); break;
+      case 10: launch_hist_t<10>(n_items, item_base, act, bnd, tree, nF, nT); break;
+      case 16: launch_hist_t<16>(n_items, item_base, act, bnd, tree, nF, nT); break;
+      case 20: launch_hist_t<20>(n_items, item_base, act, bnd, tree, nF, nT); break;
+      case 24: launch_hist_t<24>(n_items, item_base, act, bnd, tree, nF, nT); break;
+      case 32: launch_hist_t<32>(n_items, item_base, act, bnd, tree, nF, nT); break;
+      case 48: launch_hist_t<48>(n_items, item_base, act, bnd, tree, nF, nT); break;
+      default: launch_hist_t<64>(n_items, item_base, act, bnd, tree, nF, nT); break;
+    }
+  }
+
+  // Histogram launch(es) of the slots [s0, s1) (slice-local index q = slot -
+  // s0).  With want_h every slot gets its H block (block q).  Otherwise a
+  // slot whose samples fit one chunk is searched inside the kernel (fused)
+  // and needs no H; a slot with several chunks adds into H block hslots-index.
+  // tot: the slice's per-class node totals [q][nC].  Returns the number of
+  // kernel launches; hslots receives the slots that have an H block, in
+  // block order.
+  uint32_t hist_slice(const std::vector<SlotInfo> &slots, size_t s0, size_t s1, const uint4 *act,
+                      const pd_train_params &P, const Bounds<F> &bnd, uint32_t tree, int mode, uint32_t ch,
+                      const std::vector<uint32_t> &tot, bool want_h, std::vector<uint32_t> &hslots,
+                      const Event *ev_begin = nullptr, const Event *ev_end = nullptr)
+  {
+    const uint32_t nF = P.n_features, nT = P.n_thresholds;
+    const uint32_t ftiles = (nF + 63) / 64;
+    const size_t per = (size_t)nC * nT * nF;
+    const uint32_t ns = (uint32_t)(s1 - s0);
+    std::vector<HistItem> items;
+    std::vector<uint32_t> item_start(ns + 1, 0);
+    std::vector<int32_t> snode(ns), shidx(ns, -1);
+    std::vector<uint8_t> smode(ns, 0);
+    std::vector<uint32_t> zero, group_start;
+    hslots.clear();
+    for (uint32_t q = 0; q < ns; q++) {
+      const SlotInfo &si = slots[s0 + q];
+      snode[q] = si.node;
+      item_start[q] = (uint32_t)items.size();
+      const uint32_t cnt = si.trainable ? si.count : 0u;
+      const uint32_t chunks = std::max<uint32_t>(1u, (cnt + ch - 1) / ch);
+      if (want_h) {
+        shidx[q] = (int32_t)q;
+        hslots.push_back(q);
+        if (chunks > 1) {
+          smode[q] = SLOT_ATOMIC;
+          zero.push_back(q);
+        }
+      } else if (chunks == 1) {
+        smode[q] = SLOT_FUSED;
+      } else {
+        smode[q] = SLOT_ATOMIC;
+        shidx[q] = (int32_t)hslots.size();
+        zero.push_back((uint32_t)hslots.size());
+        hslots.push_back(q);
+      }
+      for (uint32_t k = 0; k < chunks; k++) {
+        const uint32_t b = si.begin + k * ch;
+        const uint32_t c = cnt ? std::min(ch, cnt - k * ch) : 0u;
+        group_start.push_back((uint32_t)items.size());
+        for (uint32_t ft = 0; ft < ftiles; ft++) items.push_back(HistItem{q, b, c, ft});
+      }
+    }
+    item_start[ns] = (uint32_t)items.size();
+    group_start.push_back((uint32_t)items.size());
+    if (mode == PD_MODE_LEVEL_BATCHED) xcd_order(items, group_start);
+    d_items.ensure(std::max<size_t>(items.size(), 1) * sizeof(HistItem));
+    d_slot_node.ensure(ns * sizeof(int32_t));
+    d_slot_mode.ensure(ns);
+    d_slot_hidx.ensure(ns * sizeof(int32_t));
+    d_tot.ensure((size_t)ns * nC * sizeof(uint32_t));
+    d_H.ensure(per * std::max<size_t>(hslots.size(), 1) * sizeof(uint32_t));
+    d_cand_key.ensure((size_t)ns * ftiles * sizeof(unsigned long long));
+    d_cand_left.ensure((size_t)ns * ftiles * nC * sizeof(uint32_t));
+    HIPCHK(hipMemcpyAsync(d_items.p, items.data(), items.size() * sizeof(HistItem), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(d_slot_node.p, snode.data(), ns * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(d_slot_mode.p, smode.data(), ns, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(d_slot_hidx.p, shidx.data(), ns * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(d_tot.p, tot.data(), (size_t)ns * nC * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+    if (!zero.empty()) {
+      d_zero.ensure(zero.size() * sizeof(uint32_t));
+      HIPCHK(hipMemcpyAsync(d_zero.p, zero.data(), zero.size() * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+    }
+    if (ev_begin) HIPCHK(hipEventRecord(ev_begin->e, stream));
+    if (!zero.empty()) {
+      const size_t total = zero.size() * per;
+      const uint32_t blocks = (uint32_t)std::min<size_t>((total + 255) / 256, (size_t)cu_count * 64);
+      hipLaunchKernelGGL(k_zero_slots, dim3(blocks), dim3(256), 0, stream, d_H.as<uint32_t>(),
+                         d_zero.as<uint32_t>(), (uint32_t)zero.size(), per);
+      HIPCHK(hipGetLastError());
+    }
+    uint32_t launches = 0;
+    if (mode == PD_MODE_LEVEL_BATCHED) {
+      launch_hist((uint32_t)items.size(), 0, act, bnd, tree, nF, nT);
+      launches += items.empty() ? 0 : 1;
+    } else {
+      for (uint32_t q = 0; q < ns; q++) {
+        const uint32_t n = item_start[q + 1] - item_start[q];
+        if (!n) continue;
+        launch_hist(n, item_start[q], act, bnd, tree, nF, nT);
+        launches++;
+      }
+    }
+    if (ev_end) HIPCHK(hipEventRecord(ev_end->e, stream));
+    return launches;
+  }
+
+  // Split search over the H blocks: block i belongs to slot hlist[i].
+  template <int NCM>
+  void launch_split_t(uint32_t nh, const uint32_t *d_list, uint32_t nT, uint32_t nF, int mode, float *all_gains)
+  {
+    const uint32_t npairs = nT * nF;
+    const uint32_t ppb = SPLIT_THREADS * 8;
+    const uint32_t bps = (npairs + ppb - 1) / ppb;
+    const size_t per = (size_t)nC * npairs;
+    if (mode == PD_MODE_LEVEL_BATCHED) {
+      hipLaunchKernelGGL((k_split<NCM>), dim3(nh * bps), dim3(SPLIT_THREADS), 0, stream, d_H.as<uint32_t>(),
+                         d_tot.as<uint32_t>(), nC, nT, nF, bps, ppb, d_best.as<unsigned long long>(), all_gains,
+                         d_list);
+      HIPCHK(hipGetLastError());
+    } else {
+      for (uint32_t i = 0; i < nh; i++) {
+        hipLaunchKernelGGL((k_split<NCM>), dim3(bps), dim3(SPLIT_THREADS), 0, stream, d_H.as<uint32_t>() + i * per,
+                           d_tot.as<uint32_t>(), nC, nT, nF, bps, ppb, d_best.as<unsigned long long>(), all_gains,
+                           d_list + i);
+        HIPCHK(hipGetLastError());
+      }
+    }
+  }
+
+  // Split search + finalize of a slice of ns slots whose H blocks are
+  // hslots (the fused slots were searched by the histogram kernel);
+  // res_out[q*(nC+2) + {0: k, 1: gain bits, 2..: left counts}].
+  void split_slice(uint32_t ns, const std::vector<uint32_t> &hslots, uint32_t nT, uint32_t nF, int mode,
+                   std::vector<uint32_t> &res_out, float *all_gains = nullptr)
+  {
+    const uint32_t nh = (uint32_t)hslots.size();
+    d_best.ensure((size_t)ns * sizeof(uint64_t));
+    d_res.ensure((size_t)ns * (nC + 2) * sizeof(uint32_t));
+    d_hlist.ensure(std::max<size_t>(nh, 1) * sizeof(uint32_t));
+    HIPCHK(hipMemsetAsync(d_best.p, 0, (size_t)ns * sizeof(uint64_t), stream));
+    if (nh) {
+      HIPCHK(hipMemcpyAsync(d_hlist.p, hslots.data(), nh * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+      const uint32_t *dl = d_hlist.as<uint32_t>();
+      if (nC <= 8) launch_split_t<8>(nh, dl, nT, nF, mode, all_gains);
+      else if (nC <= 16) launch_split_t<16>(nh, dl, nT, nF, mode, all_gains);
+      else if (nC <= 32) launch_split_t<32>(nh, dl, nT, nF, mode, all_gains);
+      else launch_split_t<64>(nh, dl, nT, nF, mode, all_gains);
+    }
+    const uint32_t threads = ns * (nC + 2);
+    hipLaunchKernelGGL(k_finalize, dim3((threads + 255) / 256), dim3(256), 0, stream, d_H.as<uint32_t>(),
+                       d_best.as<unsigned long long>(), d_slot_mode.as<uint8_t>(), d_slot_hidx.as<int32_t>(),
+                       d_cand_key.as<unsigned long long>(), d_cand_left.as<uint32_t>(), ns, nC, nT, nF,
+                       d_res.as<uint32_t>());
+    HIPCHK(hipGetLastError());
+    res_out.resize((size_t)ns * (nC + 2));
+    HIPCHK(hipMemcpyAsync(res_out.data(), d_res.p, res_out.size() * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                          stream));
+    HIPCHK(hipStreamSynchronize(stream));
+  }
+
+  void train(const pd_train_params &P, uint32_t tree_id, uint32_t depth, uint32_t end_depth, int mode,
+             const float *prior, int32_t *left, void *feats_v, void *thrs_v, float *post, uint32_t *node_k,
+             float *node_gain, double *node_gain64, pd_level_stats *ls) override
+  {
+    if (!S) throw StateError("pd_trainer_train before pd_trainer_set_data");
+    check_params(P, FD, sizeof(T) == 2);
+    require(depth >= 1 && depth <= 24, "tree depth must be in [1, 24]");
+    require(end_depth >= 1 && end_depth <= depth, "end_depth must be in [1, tree depth]");
+    require(mode == PD_MODE_PER_NODE || mode == PD_MODE_LEVEL_BATCHED, "unknown training mode");
+    require(nC <= 64, "at most 64 classes");
+    require(prior && left && feats_v && thrs_v && post, "null tree array");
+    set_device(device);
+    T *feats = (T *)feats_v;
+    T *thrs = (T *)thrs_v;
+    const Bounds<F> bnd = make_bounds<F>(P);
+    const uint32_t nF = P.n_features, nT = P.n_thresholds, thr = P.per_leaf_samples_thr;
+    const size_t N = ((size_t)1 << depth) - 1;
+    // Tree::_init (tree_impl.hpp:239-273), root posterior = priors
+    std::fill(left, left + N, -2);
+    std::fill(feats, feats + N * FD, (T)0);
+    std::fill(thrs, thrs + N, (T)0);
+    std::fill(post, post + N * nC, 0.0f);
+    std::copy(prior, prior + nC, post);
+    std::vector<uint32_t> nodeTot(N, 0u), classTot(N * nC, 0u);
+    nodeTot[0] = (uint32_t)S;
+    std::copy(root_tot.begin(), root_tot.end(), classTot.begin());
+
+    d_tleft.ensure(N * sizeof(int32_t));
+    d_tfeat.ensure(N * FD * sizeof(T));
+    d_tthr.ensure(N * sizeof(T));
+    d_actA.ensure(S * sizeof(uint4));
+    d_actB.ensure(S * sizeof(uint4));
+    d_slotA.ensure(S * sizeof(uint32_t));
+    d_slotB.ensure(S * sizeof(uint32_t));
+    HIPCHK(hipMemcpyAsync(d_actA.p, d_recs.p, S * sizeof(uint4), hipMemcpyDeviceToDevice, stream));
+    HIPCHK(hipMemsetAsync(d_slotA.p, 0, S * sizeof(uint32_t), stream));
+    uint4 *act = d_actA.as<uint4>(), *act_next = d_actB.as<uint4>();
+    uint32_t *aslot = d_slotA.as<uint32_t>(), *aslot_next = d_slotB.as<uint32_t>();
+    uint64_t n_act = S;
+
+    std::vector<SlotInfo> slots{SlotInfo{0, 0u, (uint32_t)S, S > thr}};
+    Event e0, e1, e2, e3, e4;
+    const size_t per = (size_t)nC * nT * nF;
+    const size_t per_bytes = per * sizeof(uint32_t);
+    const size_t h_per_slice = std::max<size_t>(1, hist_budget / per_bytes);   // H blocks per slice
+
+    for (uint32_t d = 1; d < end_depth; d++) {
+      const auto t_start = std::chrono::steady_clock::now();
+      pd_level_stats st{};
+      st.level = d;
+      st.frontier_nodes = (uint32_t)slots.size();
+      if (slots.empty()) {
+        if (ls) ls[d - 1] = st;
+        continue;
+      }
+      uint64_t active = 0;
+      for (const SlotInfo &s : slots)
+        if (s.trainable) active += s.count;
+      st.active_samples = active;
+      st.evals = (double)active * nF * nT;
+      uint32_t ch = pick_chunk(active, (nF + 63) / 64);
+      std::vector<uint8_t> split_flag(slots.size(), 0);
+      for (size_t s0 = 0; s0 < slots.size();) {
+        // a slice holds at most h_per_slice multi-chunk slots (the only ones
+        // that need an H block)
+        size_t s1 = s0, nh = 0;
+        while (s1 < slots.size()) {
+          const uint32_t cnt = slots[s1].trainable ? slots[s1].count : 0u;
+          const bool needs_h = cnt > ch;
+          if (needs_h && nh == h_per_slice) break;
+          nh += needs_h ? 1 : 0;
+          s1++;
+        }
+        const uint32_t ns = (uint32_t)(s1 - s0);
+        st.slices++;
+        std::vector<uint32_t> tot((size_t)ns * nC);
+        for (uint32_t q = 0; q < ns; q++)
+          std::copy(&classTot[(size_t)slots[s0 + q].node * nC], &classTot[(size_t)slots[s0 + q].node * nC] + nC,
+                    &tot[(size_t)q * nC]);
+        std::vector<uint32_t> hslots, res;
+        st.hist_launches += hist_slice(slots, s0, s1, act, P, bnd, tree_id, mode, ch, tot, false, hslots, &e0, &e1);
+        split_slice(ns, hslots, nT, nF, mode, res);
+        HIPCHK(hipEventRecord(e2.e, stream));
+        HIPCHK(hipEventSynchronize(e2.e));
+        st.hist_ms += elapsed_ms(e0, e1);
+        st.split_ms += elapsed_ms(e1, e2);
+        // node update, cl_tree_trainer_impl_learn_best_featthr.hpp:101-233
+        for (uint32_t q = 0; q < ns; q++) {
+          const uint32_t n = (uint32_t)slots[s0 + q].node;
+          const uint32_t *r = &res[(size_t)q * (nC + 2)];
+          const uint32_t k = r[0];
+          float g;
+          std::memcpy(&g, &r[1], 4);
+          const uint32_t *lh = r + 2;
+          const uint32_t *tt = &classTot[(size_t)n * nC];
+          if (node_k) node_k[n] = k;
+          if (node_gain) node_gain[n] = g;
+          if (node_gain64) node_gain64[n] = gain_f64(lh, tt, nC);
+          uint32_t lS = 0, rS = 0;
+          for (uint32_t c = 0; c < nC; c++) {
+            lS += lh[c];
+            rS += tt[c] - lh[c];
+          }
+          if (!lS || !rS) continue;
+          const uint32_t bt = k / nF, bf = k % nF;
+          const size_t L = 2 * (size_t)n + 1, R = L + 1;
+          nodeTot[L] = lS;
+          nodeTot[R] = rS;
+          for (uint32_t c = 0; c < nC; c++) {
+            classTot[L * nC + c] = lh[c];
+            classTot[R * nC + c] = tt[c] - lh[c];
+            post[L * nC + c] = (float)lh[c] / (float)lS;
+            post[R * nC + c] = (float)(tt[c] - lh[c]) / (float)rS;
+          }
+          left[L] = -1;
+          left[R] = -1;
+          candidate_params<T, FD>(tree_id, n, bf, bnd.lo, bnd.hi, &feats[(size_t)n * FD]);
+          thrs[n] = candidate_threshold<T>(tree_id, n, bf, bt, bnd.tlo, bnd.thi);
+          left[n] = (int32_t)L;
+          split_flag[s0 + q] = 1;
+          st.split_nodes++;
+        }
+        s0 = s1;
+      }
+      if (d + 1 >= end_depth) {
+        st.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+        if (ls) ls[d - 1] = st;
+        break;
+      }
+      // upload this level's node range for routing
+      const size_t lv0 = ((size_t)1 << (d - 1)) - 1, lvn = (size_t)1 << (d - 1);
+      HIPCHK(hipMemcpyAsync(d_tleft.as<int32_t>() + lv0, left + lv0, lvn * sizeof(int32_t), hipMemcpyHostToDevice,
+                            stream));
+      HIPCHK(hipMemcpyAsync(d_tfeat.as<T>() + lv0 * FD, feats + lv0 * FD, lvn * FD * sizeof(T),
+                            hipMemcpyHostToDevice, stream));
+      HIPCHK(hipMemcpyAsync(d_tthr.as<T>() + lv0, thrs + lv0, lvn * sizeof(T), hipMemcpyHostToDevice, stream));
+      // next frontier (children with more than perLeafSamplesThr samples,
+      // _impl_init.hpp:343-355), in node order
+      std::vector<SlotInfo> next;
+      std::vector<int32_t> snext(2 * slots.size(), -1), rnode(slots.size());
+      std::vector<uint32_t> nbegin;   // first entry of each next slot
+      uint32_t run = 0;
+      for (size_t q = 0; q < slots.size(); q++) {
+        rnode[q] = slots[q].node;
+        if (!split_flag[q]) continue;
+        for (int side = 0; side < 2; side++) {
+          const int32_t ch_node = 2 * slots[q].node + 1 + side;
+          const uint32_t cnt = nodeTot[ch_node];
+          if (cnt > thr) {
+            snext[2 * q + side] = (int32_t)next.size();
+            next.push_back(SlotInfo{ch_node, run, cnt, true});
+            nbegin.push_back(run);
+            run += cnt;
+          }
+        }
+      }
+      const uint32_t nq = (uint32_t)slots.size();
+      std::vector<uint32_t> sbegin(nq + 1);
+      for (uint32_t q = 0; q < nq; q++) sbegin[q] = slots[q].begin;
+      sbegin[nq] = (uint32_t)n_act;
+      d_rslot_node.ensure(nq * sizeof(int32_t));
+      d_slot_next.ensure(2 * (size_t)nq * sizeof(int32_t));
+      d_ccount.ensure(2 * (size_t)nq * sizeof(uint32_t));
+      d_cursor.ensure(std::max<size_t>(nbegin.size(), 1) * sizeof(uint32_t));
+      d_sbegin.ensure((nq + 1) * sizeof(uint32_t));
+      d_flags.ensure((n_act + 1) * sizeof(unsigned long long));
+      d_rank.ensure((n_act + 1) * sizeof(unsigned long long));
+      HIPCHK(hipMemcpyAsync(d_rslot_node.p, rnode.data(), nq * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+      HIPCHK(hipMemcpyAsync(d_slot_next.p, snext.data(), 2 * (size_t)nq * sizeof(int32_t), hipMemcpyHostToDevice,
+                            stream));
+      HIPCHK(hipMemcpyAsync(d_sbegin.p, sbegin.data(), (nq + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+      if (!nbegin.empty())
+        HIPCHK(hipMemcpyAsync(d_cursor.p, nbegin.data(), nbegin.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                              stream));
+      size_t tmp_bytes = 0;
+      HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, d_flags.as<unsigned long long>(),
+                                              d_rank.as<unsigned long long>(), (int)(n_act + 1), stream));
+      d_scan_tmp.ensure(tmp_bytes);
+      HIPCHK(hipEventRecord(e3.e, stream));
+      const uint32_t nb = (uint32_t)((n_act + 1 + 255) / 256);
+      hipLaunchKernelGGL((k_route_flags<F>), dim3(nb), dim3(256), 0, stream, d_pixels.as<I>(), d_imgs.as<ImgDesc>(),
+                         act, aslot, (uint32_t)n_act, d_rslot_node.as<int32_t>(), d_tleft.as<int32_t>(),
+                         d_tfeat.as<T>(), d_tthr.as<T>(), d_flags.as<unsigned long long>());
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipcub::DeviceScan::ExclusiveSum(d_scan_tmp.p, tmp_bytes, d_flags.as<unsigned long long>(),
+                                              d_rank.as<unsigned long long>(), (int)(n_act + 1), stream));
+      if (n_act) {
+        hipLaunchKernelGGL(k_route_scatter, dim3((uint32_t)((n_act + 255) / 256)), dim3(256), 0, stream, act, aslot,
+                           (uint32_t)n_act, d_flags.as<unsigned long long>(), d_rank.as<unsigned long long>(),
+                           d_sbegin.as<uint32_t>(), d_slot_next.as<int32_t>(), d_cursor.as<uint32_t>(), act_next,
+                           aslot_next, (uint32_t)S);
+        HIPCHK(hipGetLastError());
+      }
+      hipLaunchKernelGGL(k_route_counts, dim3((nq + 255) / 256), dim3(256), 0, stream, d_rank.as<unsigned long long>(),
+                         d_sbegin.as<uint32_t>(), nq, d_ccount.as<uint32_t>());
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipEventRecord(e4.e, stream));
+      std::vector<uint32_t> ccount(2 * (size_t)nq);
+      HIPCHK(hipMemcpyAsync(ccount.data(), d_ccount.p, ccount.size() * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                            stream));
+      HIPCHK(hipStreamSynchronize(stream));
+      st.route_ms = elapsed_ms(e3, e4);
+      // the routed child populations must equal the split's class counts
+      for (size_t q = 0; q < slots.size(); q++) {
+        if (!split_flag[q]) continue;
+        const int32_t n = slots[q].node;
+        if (ccount[2 * q] != nodeTot[2 * n + 1] || ccount[2 * q + 1] != nodeTot[2 * n + 2])
+          throw ParityError("routing of node " + std::to_string(n) + " disagrees with its histogram split");
+      }
+      std::swap(act, act_next);
+      std::swap(aslot, aslot_next);
+      n_act = run;
+      slots.swap(next);
+      st.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+      if (ls) ls[d - 1] = st;
+    }
+  }
+
+  void hist_nodes(const pd_train_params &P, uint32_t tree_id, uint32_t n_nodes, const int32_t *nodes,
+                  const int32_t *sample_node, uint32_t *H_out) override
+  {
+    if (!S) throw StateError("pd_trainer_hist_nodes before pd_trainer_set_data");
+    check_params(P, FD, sizeof(T) == 2);
+    require(n_nodes >= 1 && nodes && sample_node && H_out, "bad hist_nodes arguments");
+    set_device(device);
+    std::map<int32_t, uint32_t> slot_of;
+    for (uint32_t q = 0; q < n_nodes; q++) {
+      require(slot_of.count(nodes[q]) == 0, "duplicate node id");
+      slot_of[nodes[q]] = q;
+    }
+    std::vector<std::vector<uint4>> per(n_nodes);
+    for (uint64_t s = 0; s < S; s++) {
+      auto itn = slot_of.find(sample_node[s]);
+      if (itn != slot_of.end()) per[itn->second].push_back(h_recs[s]);
+    }
+    std::vector<uint4> act;
+    std::vector<SlotInfo> slots;
+    for (uint32_t q = 0; q < n_nodes; q++) {
+      slots.push_back(SlotInfo{nodes[q], (uint32_t)act.size(), (uint32_t)per[q].size(), true});
+      act.insert(act.end(), per[q].begin(), per[q].end());
+    }
+    d_actA.ensure(std::max<size_t>(act.size(), 1) * sizeof(uint4));
+    if (!act.empty())
+      HIPCHK(hipMemcpyAsync(d_actA.p, act.data(), act.size() * sizeof(uint4), hipMemcpyHostToDevice, stream));
+    const Bounds<F> bnd = make_bounds<F>(P);
+    const uint32_t ch = pick_chunk(act.size(), (P.n_features + 63) / 64);
+    std::vector<uint32_t> tot((size_t)n_nodes * nC, 0u), hslots;
+    hist_slice(slots, 0, slots.size(), d_actA.as<uint4>(), P, bnd, tree_id, PD_MODE_LEVEL_BATCHED, ch, tot, true,
+               hslots);
+    const size_t per_n = (size_t)nC * P.n_thresholds * P.n_features;
+    HIPCHK(hipMemcpyAsync(H_out, d_H.p, per_n * n_nodes * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+  }
+};
+
+inline TrainerBase *make_trainer(int kind, int device, uint32_t nC)
+{
+  switch (kind) {
+    case PD_FEAT_DEPTH_F32: return new Trainer<DepthF32Feature>(device, nC);
+    case PD_FEAT_DEPTH_U16: return new Trainer<DepthU16Feature>(device, nC);
+    case PD_FEAT_RGBD_U16X4: return new Trainer<RGBDFeature>(device, nC);
+    case PD_FEAT_DEMO_S16: return new Trainer<DemoS16Feature>(device, nC);
+  }
+  throw ArgError("unknown feature kind");
+}
+
+// ---------------------------------------------------------------------------
+// Forest
+// ---------------------------------------------------------------------------
+struct ForestBase {
+  int device = 0;
+  uint32_t nC = 0;
+  hipStream_t stream = nullptr;
+  virtual ~ForestBase()
+  {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  virtual void add_tree(uint32_t depth, const int32_t *left, const void *feats, const void *thrs,
+                        const float *posts) = 0;
+  virtual int n_trees() const = 0;
+  virtual void predict_host(int tree_index, const void *image, uint32_t w, uint32_t h, const uint8_t *mask,
+                            int32_t *leaves, float *post) = 0;
+  virtual void predict_frames(uint32_t n_frames, const void *d_pixels, const uint64_t *img_off,
+                              const uint32_t *widths, const uint32_t *heights, const uint8_t *d_masks,
+                              const uint64_t *mask_off, float *d_post, const uint64_t *post_off, int32_t *d_leaves,
+                              const uint64_t *leaf_off, int use_graph, float *kernel_ms) = 0;
+};
+
+template <class F>
+struct Forest final : ForestBase {
+  using T = typename F::FeatType;
+  using I = typename F::ImgType;
+  using Rec = NodeRec<T, F::FeatDim>;
+  std::vector<Rec> h_nodes;
+  std::vector<float> h_posts;
+  std::vector<uint64_t> h_tree_off, h_post_off;
+  std::vector<uint32_t> depths;
+  uint32_t max_depth = 1;
+  DevBuf d_nodes, d_posts, d_tree_off, d_post_off;
+  DevBuf d_img, d_mask, d_out, d_leaf, d_frames;
+  // captured graph for predict_frames (keyed by the launch arguments)
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  std::vector<uint64_t> gkey;
+
+  Forest(int dev, uint32_t ncl)
+  {
+    device = dev;
+    nC = ncl;
+    set_device(device);
+    HIPCHK(hipStreamCreate(&stream));
+  }
+  ~Forest() override
+  {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+  int n_trees() const override { return (int)depths.size(); }
+
+  void add_tree(uint32_t depth, const int32_t *left, const void *feats_v, const void *thrs_v,
+                const float *posts) override
+  {
+    require(depth >= 1 && depth <= 24, "tree depth must be in [1, 24]");
+    require(left && feats_v && thrs_v && posts, "null tree array");
+    const T *feats = (const T *)feats_v;
+    const T *thrs = (const T *)thrs_v;
+    const size_t N = ((size_t)1 << depth) - 1;
+    // every split node must point at its own children (2i+1, 2i+2 < N)
+    for (size_t i = 0; i < N; i++)
+      if (left[i] >= 0) require((size_t)left[i] == 2 * i + 1 && 2 * i + 2 < N, "malformed tree: bad leftChild");
+    set_device(device);
+    h_tree_off.push_back(h_nodes.size());
+    h_post_off.push_back(h_posts.size());
+    for (size_t i = 0; i < N; i++) {
+      Rec r{};
+      r.left = left[i];
+      r.thr = thrs[i];
+      for (int j = 0; j < F::FeatDim; j++) r.p[j] = feats[i * F::FeatDim + j];
+      h_nodes.push_back(r);
+    }
+    h_posts.insert(h_posts.end(), posts, posts + N * nC);
+    depths.push_back(depth);
+    max_depth = std::max(max_depth, depth);
+    d_nodes.ensure(h_nodes.size() * sizeof(Rec));
+    d_posts.ensure(h_posts.size() * sizeof(float));
+    d_tree_off.ensure(h_tree_off.size() * sizeof(uint64_t));
+    d_post_off.ensure(h_post_off.size() * sizeof(uint64_t));
+    HIPCHK(hipMemcpy(d_nodes.p, h_nodes.data(), h_nodes.size() * sizeof(Rec), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_posts.p, h_posts.data(), h_posts.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_tree_off.p, h_tree_off.data(), h_tree_off.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_post_off.p, h_post_off.data(), h_post_off.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+    if (gexec) {
+      (void)hipGraphExecDestroy(gexec);
+      gexec = nullptr;
+    }
+    if (graph) {
+      (void)hipGraphDestroy(graph);
+      graph = nullptr;
+    }
+    gkey.clear();
+  }
+
+  template <int NCM>
+  void launch(uint32_t blocks, const I *pix, const FrameDesc *frames, uint32_t n_frames, const uint8_t *masks,
+              uint32_t tree0, uint32_t ntr, uint32_t steps, float *out_post, int32_t *out_leaf)
+  {
+    hipLaunchKernelGGL((k_predict<F, NCM>), dim3(blocks), dim3(256), 0, stream, pix, frames, n_frames, masks,
+                       d_nodes.as<Rec>(), d_tree_off.as<uint64_t>() + tree0, ntr, steps, d_posts.as<float>(),
+                       d_post_off.as<uint64_t>() + tree0, nC, out_post, out_leaf);
+    HIPCHK(hipGetLastError());
+  }
+  void launch_any(uint32_t blocks, const I *pix, const FrameDesc *frames, uint32_t n_frames, const uint8_t *masks,
+                  uint32_t tree0, uint32_t ntr, uint32_t steps, float *out_post, int32_t *out_leaf)
+  {
+    if (nC <= 8) launch<8>(blocks, pix, frames, n_frames, masks, tree0, ntr, steps, out_post, out_leaf);
+    else if (nC <= 16) launch<16>(blocks, pix, frames, n_frames, masks, tree0, ntr, steps, out_post, out_leaf);
+    else if (nC <= 32) launch<32>(blocks, pix, frames, n_frames, masks, tree0, ntr, steps, out_post, out_leaf);
+    else launch<64>(blocks, pix, frames, n_frames, masks, tree0, ntr, steps, out_post, out_leaf);
+  }
+
+  void predict_host(int tree_index, const void *image, uint32_t w, uint32_t h, const uint8_t *mask,
+                    int32_t *leaves, float *post) override
+  {
+    require(!depths.empty(), "forest has no trees");
+    require(image && w >= 1 && h >= 1 && w < 65536 && h < 65536, "bad image");
+    set_device(device);
+    const size_t npx = (size_t)w * h;
+    d_img.ensure(npx * F::nChannels * sizeof(I));
+    HIPCHK(hipMemcpyAsync(d_img.p, image, npx * F::nChannels * sizeof(I), hipMemcpyHostToDevice, stream));
+    if (mask) {
+      d_mask.ensure(npx);
+      HIPCHK(hipMemcpyAsync(d_mask.p, mask, npx, hipMemcpyHostToDevice, stream));
+    }
+    FrameDesc fdsc{};
+    fdsc.w = w;
+    fdsc.h = h;
+    d_frames.ensure(sizeof(FrameDesc));
+    HIPCHK(hipMemcpyAsync(d_frames.p, &fdsc, sizeof(FrameDesc), hipMemcpyHostToDevice, stream));
+    const uint32_t blocks = (uint32_t)((npx + 255) / 256);
+    if (leaves) {
+      require(tree_index >= 0 && tree_index < (int)depths.size(), "tree index out of range");
+      d_leaf.ensure(npx * sizeof(int32_t));
+      launch_any(blocks, d_img.as<I>(), d_frames.as<FrameDesc>(), 1, mask ? d_mask.as<uint8_t>() : nullptr,
+                 (uint32_t)tree_index, 1, depths[tree_index] - 1, nullptr, d_leaf.as<int32_t>());
+      HIPCHK(hipMemcpyAsync(leaves, d_leaf.p, npx * sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+    }
+    if (post) {
+      d_out.ensure(npx * nC * sizeof(float));
+      launch_any(blocks, d_img.as<I>(), d_frames.as<FrameDesc>(), 1, mask ? d_mask.as<uint8_t>() : nullptr, 0,
+                 (uint32_t)depths.size(), max_depth - 1, d_out.as<float>(), nullptr);
+      HIPCHK(hipMemcpyAsync(post, d_out.p, npx * nC * sizeof(float), hipMemcpyDeviceToHost, stream));
+    }
+    HIPCHK(hipStreamSynchronize(stream));
+  }
+
+  void predict_frames(uint32_t n_frames, const void *d_pixels, const uint64_t *img_off, const uint32_t *widths,
+                      const uint32_t *heights, const uint8_t *d_masks, const uint64_t *mask_off, float *d_post,
+                      const uint64_t *post_off, int32_t *d_leaves, const uint64_t *leaf_off, int use_graph,
+                      float *kernel_ms) override
+  {
+    require(!depths.empty(), "forest has no trees");
+    require(n_frames >= 1 && d_pixels && img_off && widths && heights, "bad frame arguments");
+    require(d_post || d_leaves, "no output requested");
+    require(!d_post || post_off, "post_offsets missing");
+    require(!d_leaves || leaf_off, "leaf_offsets missing");
+    require(!d_masks || mask_off, "mask_offsets missing");
+    set_device(device);
+    std::vector<FrameDesc> fr(n_frames);
+    uint32_t blocks = 0;
+    for (uint32_t i = 0; i < n_frames; i++) {
+      require(widths[i] >= 1 && heights[i] >= 1 && widths[i] < 65536 && heights[i] < 65536, "bad frame size");
+      fr[i].img_off = img_off[i];
+      fr[i].post_off = d_post ? post_off[i] : 0;
+      fr[i].leaf_off = d_leaves ? leaf_off[i] : 0;
+      fr[i].mask_off = d_masks ? mask_off[i] : 0;
+      fr[i].w = widths[i];
+      fr[i].h = heights[i];
+      fr[i].block0 = blocks;
+      blocks += (widths[i] * heights[i] + 255) / 256;
+    }
+    std::vector<uint64_t> key;
+    key.push_back((uint64_t)(uintptr_t)d_pixels);
+    key.push_back((uint64_t)(uintptr_t)d_masks);
+    key.push_back((uint64_t)(uintptr_t)d_post);
+    key.push_back((uint64_t)(uintptr_t)d_leaves);
+    for (const FrameDesc &f : fr) {
+      key.push_back(f.img_off);
+      key.push_back(f.post_off);
+      key.push_back(f.leaf_off);
+      key.push_back(f.mask_off);
+      key.push_back(((uint64_t)f.w << 32) | f.h);
+    }
+    const bool same = use_graph && gexec && key == gkey;
+    if (!same) {
+      d_frames.ensure(n_frames * sizeof(FrameDesc));
+      HIPCHK(hipMemcpyAsync(d_frames.p, fr.data(), n_frames * sizeof(FrameDesc), hipMemcpyHostToDevice, stream));
+      HIPCHK(hipStreamSynchronize(stream));
+    }
+    Event a, b;
+    HIPCHK(hipEventRecord(a.e, stream));
+    if (!use_graph) {
+      launch_any(blocks, (const I *)d_pixels, d_frames.as<FrameDesc>(), n_frames, d_masks, 0,
+                 (uint32_t)depths.size(), max_depth - 1, d_post, d_leaves);
+    } else {
+      if (!same) {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        gexec = nullptr;
+        graph = nullptr;
+        hipStream_t cs;
+        HIPCHK(hipStreamCreate(&cs));
+        HIPCHK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+        hipStream_t keep = stream;
+        stream = cs;
+        try {
+          launch_any(blocks, (const I *)d_pixels, d_frames.as<FrameDesc>(), n_frames, d_masks, 0,
+                     (uint32_t)depths.size(), max_depth - 1, d_post, d_leaves);
+        } catch (...) {
+          stream = keep;
+          hipGraph_t g;
+          (void)hipStreamEndCapture(cs, &g);
+          (void)hipStreamDestroy(cs);
+          throw;
+        }
+        stream = keep;
+        HIPCHK(hipStreamEndCapture(cs, &graph));
+        HIPCHK(hipStreamDestroy(cs));
+        HIPCHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+        gkey = key;
+      }
+      HIPCHK(hipGraphLaunch(gexec, stream));
+    }
+    HIPCHK(hipEventRecord(b.e, stream));
+    HIPCHK(hipEventSynchronize(b.e));
+    if (kernel_ms) *kernel_ms = (float)elapsed_ms(a, b);
+  }
+};
+
+inline ForestBase *make_forest(int kind, int device, uint32_t nC)
+{
+  switch (kind) {
+    case PD_FEAT_DEPTH_F32: return new Forest<DepthF32Feature>(device, nC);
+    case PD_FEAT_DEPTH_U16: return new Forest<DepthU16Feature>(device, nC);
+    case PD_FEAT_RGBD_U16X4: return new Forest<RGBDFeature>(device, nC);
+    case PD_FEAT_DEMO_S16: return new Forest<DemoS16Feature>(device, nC);
+  }
+  throw ArgError("unknown feature kind");
+}
+
+template <class F>
+void candidate_tables_t(const pd_train_params &P, uint32_t tree, uint32_t n_nodes, const int32_t *nodes,
+                        void *params_out, void *thr_out)
+{
+  using T = typename F::FeatType;
+  check_params(P, F::FeatDim, sizeof(T) == 2);
+  const size_t nfe = (size_t)n_nodes * P.n_features;
+  DevBuf dn, dp, dt;
+  dn.ensure(n_nodes * sizeof(int32_t));
+  dp.ensure(nfe * F::FeatDim * sizeof(T));
+  dt.ensure(nfe * P.n_thresholds * sizeof(T));
+  HIPCHK(hipMemcpy(dn.p, nodes, n_nodes * sizeof(int32_t), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL((k_candidates<F>), dim3((uint32_t)((nfe + 255) / 256)), dim3(256), 0, nullptr,
+                     tree, dn.as<int32_t>(), n_nodes, P.n_features, P.n_thresholds, make_bounds<F>(P), dp.as<T>(),
+                     dt.as<T>());
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpy(params_out, dp.p, nfe * F::FeatDim * sizeof(T), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(thr_out, dt.p, nfe * P.n_thresholds * sizeof(T), hipMemcpyDeviceToHost));
+}
+
+template <class F>
+void feature_eval_t(const void *image, uint32_t w, uint32_t h, uint32_t n, const int32_t *xy, const void *params,
+                    void *out)
+{
+  using T = typename F::FeatType;
+  using I = typename F::ImgType;
+  DevBuf di, dx, dp, dout;
+  const size_t npx = (size_t)w * h * F::nChannels;
+  di.ensure(npx * sizeof(I));
+  dx.ensure((size_t)n * 2 * sizeof(int32_t));
+  dp.ensure((size_t)n * F::FeatDim * sizeof(T));
+  dout.ensure((size_t)n * sizeof(T));
+  for (uint32_t i = 0; i < n; i++)
+    require(xy[2 * i] >= 0 && (uint32_t)xy[2 * i] < w && xy[2 * i + 1] >= 0 && (uint32_t)xy[2 * i + 1] < h,
+            "evaluation pixel outside the image");
+  HIPCHK(hipMemcpy(di.p, image, npx * sizeof(I), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dx.p, xy, (size_t)n * 2 * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dp.p, params, (size_t)n * F::FeatDim * sizeof(T), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL((k_feature_eval<F>), dim3((n + 255) / 256), dim3(256), 0, nullptr, di.as<I>(), w, h,
+                     dx.as<int32_t>(), dp.as<T>(), n, dout.as<T>());
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpy(out, dout.p, (size_t)n * sizeof(T), hipMemcpyDeviceToHost));
+}
+
+}  // namespace engine
+}  // namespace pd
+
+// Opaque handles of the C ABI.
+struct pd_trainer {
+  std::unique_ptr<pd::engine::TrainerBase> impl;
+};
+struct pd_forest {
+  std::unique_ptr<pd::engine::ForestBase> impl;
+};

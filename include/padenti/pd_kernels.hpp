@@ -25,8 +25,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
-#include "../../include/padenti/pd_math.hpp"
-#include "../../include/padenti/pd_features.hpp"
+#include "pd_math.hpp"
+#include "pd_features.hpp"
 
 #pragma clang fp contract(off)
 

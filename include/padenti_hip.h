@@ -26,6 +26,10 @@
 #include <stddef.h>
 #include <stdint.h>
 
+/* Exported symbols; the library is built with hidden default visibility so
+ * that separately compiled feature plugins never interpose each other. */
+#define PD_API __attribute__((visibility("default")))
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -83,21 +87,21 @@ typedef struct {
 typedef struct pd_trainer pd_trainer;
 typedef struct pd_forest pd_forest;
 
-const char *pd_last_error(void);
-int pd_version(void);
+PD_API const char *pd_last_error(void);
+PD_API int pd_version(void);
 /* Number of visible HIP devices (0 without a GPU; never fails). */
-int pd_device_count(void);
+PD_API int pd_device_count(void);
 
 /* ---- training (replaces CLTreeTrainer, cl_tree_trainer.hpp:115) ----- */
-int pd_trainer_create(pd_trainer **out, int device, int feature_kind, uint32_t n_classes);
-int pd_trainer_destroy(pd_trainer *tr);
+PD_API int pd_trainer_create(pd_trainer **out, int device, int feature_kind, uint32_t n_classes);
+PD_API int pd_trainer_destroy(pd_trainer *tr);
 
 /* Upload the training set (TrainingSetImage list, training_set_image.hpp:64-66).
  * pixels: all images packed, img_offsets[i] = element offset of image i;
  * labels: packed label images (uint8, 0 = unlabelled, 1..nC), label_offsets
  * in bytes; samples: packed pixel indices y*w+x, sample_offsets in elements.
  * Every sample must be labelled (the reference indexes label-1). */
-int pd_trainer_set_data(pd_trainer *tr, uint32_t n_images, const void *pixels, const uint64_t *img_offsets,
+PD_API int pd_trainer_set_data(pd_trainer *tr, uint32_t n_images, const void *pixels, const uint64_t *img_offsets,
                         const uint32_t *widths, const uint32_t *heights, const uint8_t *labels,
                         const uint64_t *label_offsets, const uint32_t *samples, const uint64_t *sample_offsets,
                         const uint32_t *n_samples);
@@ -105,9 +109,9 @@ int pd_trainer_set_data(pd_trainer *tr, uint32_t n_images, const void *pixels, c
 /* Device memory budget for histograms (bytes); frontier levels whose
  * histograms exceed it are processed in slices (the reference slices at
  * GLOBAL_HISTOGRAM_MAX_SIZE of host RAM, cl_tree_trainer_impl.hpp:50). */
-int pd_trainer_set_hist_budget(pd_trainer *tr, uint64_t bytes);
+PD_API int pd_trainer_set_hist_budget(pd_trainer *tr, uint64_t bytes);
 /* Samples per histogram work item (0 = automatic). */
-int pd_trainer_set_chunk(pd_trainer *tr, uint32_t samples_per_chunk);
+PD_API int pd_trainer_set_chunk(pd_trainer *tr, uint32_t samples_per_chunk);
 
 /* Train one tree: TreeTrainer::train(tree, ts, params, 1, end_depth)
  * (tree_trainer.hpp:82-85).  Tree arrays are the reference SoA layout of a
@@ -117,46 +121,46 @@ int pd_trainer_set_chunk(pd_trainer *tr, uint32_t samples_per_chunk);
  * per-node outputs (size N, may be NULL): chosen k = t*nF + f, FP32 gain,
  * FP64 gain of the chosen split.  level_stats (may be NULL) receives
  * end_depth-1 records. */
-int pd_trainer_train(pd_trainer *tr, const pd_train_params *params, uint32_t tree_id, uint32_t tree_depth,
+PD_API int pd_trainer_train(pd_trainer *tr, const pd_train_params *params, uint32_t tree_id, uint32_t tree_depth,
                      uint32_t end_depth, int mode, const float *root_prior, int32_t *left_children,
                      void *features, void *thresholds, float *posteriors, uint32_t *node_k, float *node_gain,
                      double *node_gain64, pd_level_stats *level_stats);
 
 /* (a) alone: histograms H[n][c][t][f] (uint32) of the listed nodes, with
  * sample_node[s] giving the node of every packed sample (-1 = none). */
-int pd_trainer_hist_nodes(pd_trainer *tr, const pd_train_params *params, uint32_t tree_id, uint32_t n_nodes,
+PD_API int pd_trainer_hist_nodes(pd_trainer *tr, const pd_train_params *params, uint32_t tree_id, uint32_t n_nodes,
                           const int32_t *node_ids, const int32_t *sample_node, uint32_t *H_out);
 
 /* (b) alone: for each node, the first maximum over k = t*nF + f of the FP32
  * gain, from host H[n][c][t][f] and per-class totals tot[n][c]. */
-int pd_split_search(int device, uint32_t n_nodes, uint32_t n_classes, uint32_t n_thresholds,
+PD_API int pd_split_search(int device, uint32_t n_nodes, uint32_t n_classes, uint32_t n_thresholds,
                     uint32_t n_features, const uint32_t *H, const uint32_t *tot, uint32_t *best_k,
                     float *best_gain, float *all_gains);
 
 /* Candidate tables on the device: params[n][f][FeatDim], thresholds[n][f][nT]. */
-int pd_candidate_tables(int device, int feature_kind, const pd_train_params *params, uint32_t tree_id,
+PD_API int pd_candidate_tables(int device, int feature_kind, const pd_train_params *params, uint32_t tree_id,
                         uint32_t n_nodes, const int32_t *node_ids, void *params_out, void *thr_out);
 
 /* Unit entry points (device evaluation of single steps). */
-int pd_md5rand(int device, uint32_t n, const uint32_t *seeds, uint32_t *out);
-int pd_log2_det(int device, uint32_t n, const float *x, float *y);
+PD_API int pd_md5rand(int device, uint32_t n, const uint32_t *seeds, uint32_t *out);
+PD_API int pd_log2_det(int device, uint32_t n, const float *x, float *y);
 /* q[i] = p[i] / d[i] through the feature hooks' per-pixel division path. */
-int pd_fast_div(int device, uint32_t n, const float *p, const float *d, float *q);
-int pd_feature_eval(int device, int feature_kind, const void *image, uint32_t w, uint32_t h, uint32_t n,
+PD_API int pd_fast_div(int device, uint32_t n, const float *p, const float *d, float *q);
+PD_API int pd_feature_eval(int device, int feature_kind, const void *image, uint32_t w, uint32_t h, uint32_t n,
                     const int32_t *xy, const void *params, void *out);
 
 /* ---- prediction (replaces CLClassifier, cl_classifier.hpp:81-94) ----- */
-int pd_forest_create(pd_forest **out, int device, int feature_kind, uint32_t n_classes);
-int pd_forest_destroy(pd_forest *fo);
+PD_API int pd_forest_create(pd_forest **out, int device, int feature_kind, uint32_t n_classes);
+PD_API int pd_forest_destroy(pd_forest *fo);
 /* CLClassifier::operator<<(const Tree&): copies the tree to the device. */
-int pd_forest_add_tree(pd_forest *fo, uint32_t depth, const int32_t *left_children, const void *features,
+PD_API int pd_forest_add_tree(pd_forest *fo, uint32_t depth, const int32_t *left_children, const void *features,
                        const void *thresholds, const float *posteriors);
-int pd_forest_n_trees(pd_forest *fo);
+PD_API int pd_forest_n_trees(pd_forest *fo);
 /* predict(id, image, leafIdx[, mask]): out[h*w] leaf ids, -1 where mask==0. */
-int pd_forest_predict_leaves(pd_forest *fo, uint32_t tree_index, const void *image, uint32_t w, uint32_t h,
+PD_API int pd_forest_predict_leaves(pd_forest *fo, uint32_t tree_index, const void *image, uint32_t w, uint32_t h,
                              const uint8_t *mask, int32_t *out);
 /* predict(image, posterior[, mask]): out[nC][h][w] planar FP32. */
-int pd_forest_predict_posterior(pd_forest *fo, const void *image, uint32_t w, uint32_t h, const uint8_t *mask,
+PD_API int pd_forest_predict_posterior(pd_forest *fo, const void *image, uint32_t w, uint32_t h, const uint8_t *mask,
                                 float *out);
 
 /* Batched multi-frame prediction on device-resident buffers (one launch,
@@ -166,7 +170,7 @@ int pd_forest_predict_posterior(pd_forest *fo, const void *image, uint32_t w, ui
  * post_offsets[i] as [nC][h][w]; d_leaves, frame i at leaf_offsets[i] as
  * [tree][h][w].  use_graph != 0 captures the launch once per shape set and
  * replays it.  *kernel_ms (may be NULL) receives the device time. */
-int pd_forest_predict_frames_dev(pd_forest *fo, uint32_t n_frames, const void *d_pixels,
+PD_API int pd_forest_predict_frames_dev(pd_forest *fo, uint32_t n_frames, const void *d_pixels,
                                  const uint64_t *img_offsets, const uint32_t *widths, const uint32_t *heights,
                                  const uint8_t *d_masks, const uint64_t *mask_offsets, float *d_post,
                                  const uint64_t *post_offsets, int32_t *d_leaves, const uint64_t *leaf_offsets,
@@ -174,14 +178,14 @@ int pd_forest_predict_frames_dev(pd_forest *fo, uint32_t n_frames, const void *d
 
 /* HBM probe: average time of a plain device-to-device copy kernel moving
  * `bytes` (read) + `bytes` (write), over `reps` launches. */
-int pd_copy_bench(int device, uint64_t bytes, int reps, float *ms_per_copy);
+PD_API int pd_copy_bench(int device, uint64_t bytes, int reps, float *ms_per_copy);
 
 /* ---- device memory helpers (for callers without another allocator) ---- */
-int pd_dev_alloc(int device, uint64_t bytes, void **out);
-int pd_dev_free(void *p);
-int pd_memcpy_h2d(void *dst, const void *src, uint64_t bytes);
-int pd_memcpy_d2h(void *dst, const void *src, uint64_t bytes);
-int pd_dev_sync(int device);
+PD_API int pd_dev_alloc(int device, uint64_t bytes, void **out);
+PD_API int pd_dev_free(void *p);
+PD_API int pd_memcpy_h2d(void *dst, const void *src, uint64_t bytes);
+PD_API int pd_memcpy_d2h(void *dst, const void *src, uint64_t bytes);
+PD_API int pd_dev_sync(int device);
 
 #ifdef __cplusplus
 }

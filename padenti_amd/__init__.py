@@ -22,11 +22,43 @@ __all__ = [
     "TrainingSetImage", "TrainingSet", "Tree", "TreeTrainerParameters", "HIPTreeTrainer",
     "HIPClassifier", "PdError", "PD_FEAT_DEPTH_F32", "PD_FEAT_DEPTH_U16", "PD_FEAT_RGBD_U16X4",
     "PD_FEAT_DEMO_S16", "PD_MODE_PER_NODE", "PD_MODE_LEVEL_BATCHED", "FEATURES", "device_count",
+    "Plugin", "example_plugin",
 ]
 
 
 def device_count():
     return lib().pd_device_count()
+
+
+class Plugin:
+    """A separately compiled feature hook (include/padenti/pd_plugin.hpp):
+    ``Plugin(path, name, img_dtype, channels, feat_dtype, feat_dim)`` loads
+    the shared library and its pd_trainer_create_<name> /
+    pd_forest_create_<name> entry points."""
+
+    def __init__(self, path, name, img_dtype, channels, feat_dtype, feat_dim):
+        import ctypes
+        self.lib = ctypes.CDLL(path)
+        self.name = name
+        self.info = (img_dtype, channels, feat_dtype, feat_dim)
+        for kind in ("trainer", "forest"):
+            fn = getattr(self.lib, "pd_%s_create_%s" % (kind, name))
+            fn.restype = ctypes.c_int
+            fn.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_uint32]
+            setattr(self, "create_" + kind, fn)
+        self.last_error = getattr(self.lib, "pd_last_error_" + name)
+        self.last_error.restype = ctypes.c_char_p
+
+    def check(self, rc):
+        if rc < 0:
+            raise PdError(rc, self.last_error().decode())
+
+
+def example_plugin():
+    """The example plugin built next to the library (float depth hook)."""
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libpd_example_plugin.so")
+    return Plugin(path, "user_depth", np.float32, 1, np.float32, 4)
 
 
 class TrainingSetImage:
@@ -122,11 +154,15 @@ class HIPTreeTrainer:
     """Replaces CLTreeTrainer(featurePath, useCPU): the feature is one of the
     compiled HIP hooks, selected by ``feature_kind``."""
 
-    def __init__(self, feature_kind, n_classes, device=0):
+    def __init__(self, feature_kind, n_classes, device=0, plugin=None):
         import ctypes
         self.kind, self.n_classes, self.device = feature_kind, int(n_classes), device
+        self.info = plugin.info if plugin is not None else FEATURES[feature_kind]
         h = ctypes.c_void_p()
-        check(lib().pd_trainer_create(ctypes.byref(h), device, feature_kind, n_classes))
+        if plugin is not None:
+            plugin.check(plugin.create_trainer(ctypes.byref(h), device, n_classes))
+        else:
+            check(lib().pd_trainer_create(ctypes.byref(h), device, feature_kind, n_classes))
         self._h = h
         self._ts = None
         self.last_level_stats = []
@@ -142,7 +178,7 @@ class HIPTreeTrainer:
     def set_training_set(self, ts):
         """Upload the training set once; later train() calls reuse it."""
         pix, img_off, w, h, labels, lab_off, samples, samp_off, ns = ts.packed()
-        want = FEATURES[self.kind][0]
+        want = self.info[0]
         pix = np.ascontiguousarray(pix, dtype=want)
         check(lib().pd_trainer_set_data(self._h, len(ts.images), ptr(pix), ptr(img_off), ptr(w), ptr(h),
                                         ptr(labels), ptr(lab_off), ptr(samples), ptr(samp_off), ptr(ns)))
@@ -189,11 +225,15 @@ class HIPClassifier:
     predict_posterior are the reference's four predict overloads
     (classifier.hpp:52-94; mask=None means every pixel)."""
 
-    def __init__(self, feature_kind, n_classes, device=0):
+    def __init__(self, feature_kind, n_classes, device=0, plugin=None):
         import ctypes
         self.kind, self.n_classes, self.device = feature_kind, int(n_classes), device
+        self.info = plugin.info if plugin is not None else FEATURES[feature_kind]
         h = ctypes.c_void_p()
-        check(lib().pd_forest_create(ctypes.byref(h), device, feature_kind, n_classes))
+        if plugin is not None:
+            plugin.check(plugin.create_forest(ctypes.byref(h), device, n_classes))
+        else:
+            check(lib().pd_forest_create(ctypes.byref(h), device, feature_kind, n_classes))
         self._h = h
 
     def close(self):
@@ -214,7 +254,7 @@ class HIPClassifier:
         return lib().pd_forest_n_trees(self._h)
 
     def _img(self, image):
-        return np.ascontiguousarray(image, dtype=FEATURES[self.kind][0])
+        return np.ascontiguousarray(image, dtype=self.info[0])
 
     def predict_leaves(self, tree_index, image, width, height, mask=None):
         img = self._img(image)

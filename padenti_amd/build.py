@@ -17,6 +17,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libpadenti_hip.so")
 SOURCES = [os.path.join(HERE, "csrc", "pd_capi.hip"), os.path.join(HERE, "csrc", "pd_synth.cpp")]
+# example of a separately compiled feature hook (include/padenti/pd_plugin.hpp)
+PLUGIN_LIB = os.path.join(HERE, "libpd_example_plugin.so")
+PLUGIN_SOURCES = [os.path.join(HERE, "examples", "depth_plugin.hip")]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HIP_FLAGS = [
@@ -29,17 +32,47 @@ HIP_FLAGS = [
     "-fno-fast-math",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",
+    "-fvisibility=hidden",
+    "-fvisibility-inlines-hidden",
     "-I" + os.path.join(ROOT, "include"),
 ]
 
 
-def build(verbose: bool = False) -> str:
-    tmp = LIB + ".tmp"
-    cmd = [HIPCC] + HIP_FLAGS + SOURCES + ["-o", tmp]
+def _compile(sources, out, verbose):
+    tmp = out + ".tmp"
+    cmd = [HIPCC] + HIP_FLAGS + sources + ["-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
-    os.replace(tmp, LIB)
+    os.replace(tmp, out)
+
+
+DEMO_SRC = os.path.join(ROOT, "tests", "cpp", "api_demo.cpp")
+DEMO_BIN = os.path.join(ROOT, "tests", "cpp", "api_demo")
+
+
+def build_demo(verbose: bool = False) -> str:
+    """C++ drop-in API demo (plain g++, links the library and the plugin)."""
+    cmd = ["g++", "-std=c++11", "-O2", "-Wall", "-I" + os.path.join(ROOT, "include"), DEMO_SRC,
+           "-L" + HERE, "-lpadenti_hip", "-lpd_example_plugin", "-Wl,-rpath,$ORIGIN/../../padenti_amd",
+           "-Wl,-rpath-link,/opt/rocm/lib", "-o", DEMO_BIN]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    return DEMO_BIN
+
+
+def build(verbose: bool = False, plugin: bool = True) -> str:
+    if plugin:
+        # the two libraries compile independently: run them side by side
+        from concurrent.futures import ThreadPoolExecutor
+        with ThreadPoolExecutor(2) as ex:
+            jobs = [ex.submit(_compile, SOURCES, LIB, verbose), ex.submit(_compile, PLUGIN_SOURCES, PLUGIN_LIB, verbose)]
+            for j in jobs:
+                j.result()
+        build_demo(verbose)
+    else:
+        _compile(SOURCES, LIB, verbose)
     return LIB
 
 

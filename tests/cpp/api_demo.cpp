@@ -1,0 +1,133 @@
+// The C++ drop-in API used the way the reference's demo uses CLTreeTrainer /
+// CLClassifier (test/test_tree_trainer.cpp:53-98, test/test_classifier.cpp):
+// build a training set, train trees with a built-in hook and with a plugin
+// hook, save / load them as XML, classify a held-out image.  Writes the
+// arrays it produced to <outdir> for the test-suite to compare with the CPU
+// oracle.   usage: api_demo <outdir>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <string>
+#include <vector>
+
+#include <padenti/padenti.hpp>
+
+extern "C" int pd_synth_image(int kind, uint64_t seed, uint32_t img, uint32_t w, uint32_t h, uint32_t n_classes,
+                              void *pixels, uint8_t *labels);
+extern "C" int pd_synth_samples(uint64_t seed, uint32_t img, const uint8_t *labels, uint32_t w, uint32_t h,
+                                uint32_t n, uint32_t *out);
+extern "C" int pd_trainer_create_user_depth(pd_trainer **out, int device, uint32_t n_classes);
+extern "C" int pd_forest_create_user_depth(pd_forest **out, int device, uint32_t n_classes);
+
+typedef TrainingSet<float, 1> TrainingSetT;
+typedef Tree<float, 4, 4> TreeT;
+typedef TreeTrainerParameters<float, 4> ParamsT;
+typedef HIPTreeTrainer<float, 1, float, 4, 4> TrainerT;
+typedef HIPClassifier<float, 1, float, 4, 4> ClassifierT;
+
+static void dump(const std::string &path, const void *p, size_t bytes)
+{
+  std::ofstream f(path.c_str(), std::ios::binary);
+  f.write((const char *)p, (std::streamsize)bytes);
+}
+
+static bool same_tree(const TreeT &a, const TreeT &b)
+{
+  const size_t n = a.getNNodes();
+  return n == b.getNNodes() && !memcmp(a.getLeftChildren(), b.getLeftChildren(), n * 4) &&
+         !memcmp(a.getFeatures(), b.getFeatures(), n * 16) && !memcmp(a.getThresholds(), b.getThresholds(), n * 4) &&
+         !memcmp(a.getPosteriors(), b.getPosteriors(), n * 16);
+}
+
+int main(int argc, char **argv)
+{
+  if (argc != 2) {
+    fprintf(stderr, "usage: %s <outdir>\n", argv[0]);
+    return 2;
+  }
+  const std::string out = argv[1];
+  const unsigned W = 160, H = 120, NS = 128, NIMG = 8;
+  TrainingSetT ts(4);
+  std::vector<float> px(W * H);
+  std::vector<uint8_t> lab(W * H);
+  std::vector<uint32_t> smp(NS);
+  for (unsigned i = 0; i < NIMG; i++) {
+    pd_synth_image(0, 0, i, W, H, 4, px.data(), lab.data());
+    pd_synth_samples(0, i, lab.data(), W, H, NS, smp.data());
+    ts << TrainingSetImage<float, 1>(px.data(), W, H, lab.data(), 4, smp.data(), NS);
+  }
+  ParamsT params;
+  params.nFeatures = 64;
+  params.nThresholds = 10;
+  params.computeFRange = false;
+  for (int j = 0; j < 4; j++) {
+    params.featLowBounds[j] = -60.0f;
+    params.featUpBounds[j] = 60.0f;
+  }
+  params.thrLowBound = -0.5f;
+  params.thrUpBound = 0.5f;
+  params.randomThrSampling = true;
+  params.perLeafSamplesThr = 10;
+
+  try {
+    TreeT tree(0, 6), tree_plugin(0, 6);
+    TrainerT trainer(PD_FEAT_DEPTH_F32);
+    trainer.train(tree, ts, params, 1, 6);
+    TrainerT trainer_plugin(&pd_trainer_create_user_depth);
+    trainer_plugin.train(tree_plugin, ts, params, 1, 6);
+    if (!same_tree(tree, tree_plugin)) {
+      fprintf(stderr, "plugin hook trained a different tree\n");
+      return 1;
+    }
+    bool threw = false;
+    try {
+      trainer.train(tree, ts, params, 2, 6);
+    } catch (const std::runtime_error &) {
+      threw = true;
+    }
+    if (!threw) return 1;
+    tree.save(out + "/tree.xml", -1, true);
+    tree.save(out + "/tree_default.xml");
+    TreeT loaded;
+    loaded.load(out + "/tree.xml");
+    if (!same_tree(tree, loaded)) {
+      fprintf(stderr, "lossless XML round trip changed the tree\n");
+      return 1;
+    }
+    const size_t n = tree.getNNodes();
+    dump(out + "/left.bin", tree.getLeftChildren(), n * 4);
+    dump(out + "/features.bin", tree.getFeatures(), n * 16);
+    dump(out + "/thresholds.bin", tree.getThresholds(), n * 4);
+    dump(out + "/posteriors.bin", tree.getPosteriors(), n * 16);
+    dump(out + "/priors.bin", ts.getPriors(), 16);
+
+    pd_synth_image(0, 0, 1000000, W, H, 4, px.data(), lab.data());
+    Image<float, 1> img(px.data(), W, H);
+    ClassifierT clf(PD_FEAT_DEPTH_F32), clf_plugin(&pd_forest_create_user_depth);
+    clf << tree;
+    clf_plugin << tree_plugin;
+    Image<float, 4> post(W, H), post_plugin(W, H);
+    Image<int, 1> leaves(W, H);
+    clf.predict(img, post);
+    clf_plugin.predict(img, post_plugin);
+    clf.predict(0, img, leaves);
+    if (memcmp(post.getData(), post_plugin.getData(), (size_t)W * H * 4 * 4)) {
+      fprintf(stderr, "plugin hook predicted differently\n");
+      return 1;
+    }
+    Image<unsigned char, 1> mask(W, H);
+    for (unsigned i = 0; i < W * H; i++) mask.getData()[i] = (unsigned char)(i % 3 != 0);
+    Image<float, 4> post_masked(W, H);
+    clf.predict(img, post_masked, mask);
+    dump(out + "/image.bin", px.data(), (size_t)W * H * 4);
+    dump(out + "/post.bin", post.getData(), (size_t)W * H * 4 * 4);
+    dump(out + "/post_masked.bin", post_masked.getData(), (size_t)W * H * 4 * 4);
+    dump(out + "/leaves.bin", leaves.getData(), (size_t)W * H * 4);
+  } catch (const std::exception &e) {
+    fprintf(stderr, "error: %s\n", e.what());
+    return 1;
+  }
+  printf("api_demo ok\n");
+  return 0;
+}

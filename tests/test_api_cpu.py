@@ -1,0 +1,63 @@
+"""Host-side checks of the drop-in API that need no GPU: the C++ headers
+compile with a plain C++11 compiler, the demo links, and the tree XML format
+round-trips."""
+import os
+import subprocess
+
+import numpy as np
+
+import padenti_amd as pa
+from padenti_amd import treeio
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_cpp_headers_compile_cxx11(tmp_path):
+    src = tmp_path / "t.cpp"
+    src.write_text("#include <padenti/padenti.hpp>\n"
+                   "typedef HIPTreeTrainer<unsigned short, 1, short, 2, 2> T;\n"
+                   "typedef HIPClassifier<unsigned short, 4, float, 6, 21> C;\n"
+                   "int main() { Tree<short, 2, 2> t(0, 3); return (int)t.getNNodes() - 7; }\n")
+    subprocess.run(["g++", "-std=c++11", "-Wall", "-Werror", "-fsyntax-only", "-I" + os.path.join(ROOT, "include"),
+                    str(src)], check=True)
+
+
+def test_cpp_demo_links():
+    from padenti_amd import build as b
+    assert os.path.exists(b.DEMO_BIN) or b.build_demo() == b.DEMO_BIN
+
+
+def _random_tree(kind, nC, depth, rng):
+    t = pa.Tree(3, depth, kind, nC)
+    t.left_children[0] = 1
+    t.left_children[1:3] = -1
+    t.left_children[1] = 3
+    t.left_children[3:5] = -1
+    t.features[[0, 1]] = rng.uniform(-60, 60, (2, t.features.shape[1])).astype(t.features.dtype)
+    t.thresholds[[0, 1]] = rng.uniform(-1, 1, 2).astype(t.thresholds.dtype)
+    t.posteriors[:5] = rng.dirichlet(np.ones(nC), 5).astype(np.float32)
+    return t
+
+
+def test_tree_xml_roundtrip(tmp_path):
+    rng = np.random.default_rng(0)
+    t = _random_tree(pa.PD_FEAT_DEPTH_F32, 4, 4, rng)
+    treeio.save(t, str(tmp_path / "a.xml"), lossless=True)
+    u = treeio.load(str(tmp_path / "a.xml"), pa.PD_FEAT_DEPTH_F32, 4, tree_id=3)
+    assert np.array_equal(t.left_children, u.left_children)
+    assert t.features.tobytes() == u.features.tobytes() and t.thresholds.tobytes() == u.thresholds.tobytes()
+    assert t.posteriors.tobytes() == u.posteriors.tobytes()
+    treeio.save(t, str(tmp_path / "b.xml"))        # reference precision: 6 significant digits
+    v = treeio.load(str(tmp_path / "b.xml"), pa.PD_FEAT_DEPTH_F32, 4, tree_id=3)
+    assert np.allclose(v.posteriors, t.posteriors, rtol=1e-5) and np.array_equal(v.left_children, t.left_children)
+    text = open(tmp_path / "b.xml").read()
+    assert text.startswith('<?xml version="1.0" encoding="utf-8"?>\n<Trees><TreeNumber>1</TreeNumber>')
+    assert "<Node2>" in text and "<Node5>" not in text    # nonexistent nodes are omitted
+
+
+def test_tree_xml_short_features(tmp_path):
+    rng = np.random.default_rng(1)
+    t = _random_tree(pa.PD_FEAT_DEMO_S16, 2, 3, rng)
+    treeio.save(t, str(tmp_path / "s.xml"))
+    u = treeio.load(str(tmp_path / "s.xml"), pa.PD_FEAT_DEMO_S16, 2, tree_id=3)
+    assert np.array_equal(t.features, u.features) and np.array_equal(t.thresholds, u.thresholds)
