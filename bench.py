@@ -70,6 +70,17 @@ def cpu_baseline(ts, params, cfg, target_s):
     }
 
 
+def traffic_per_launch(cfg_name):
+    """HBM bytes per histogram launch measured with rocprofv3 --pmc FETCH_SIZE /
+    WRITE_SIZE on this workload (scaled by the copy-kernel calibration),
+    recorded in profiles/k_hist_<cfg>_traffic.json; None if not measured."""
+    path = os.path.join(ROOT, "profiles", f"k_hist_{cfg_name}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("traffic_bytes_per_launch")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -175,7 +186,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic_per_launch(args.config),
         },
         "cpu_baseline": cpu,
     }
