@@ -1,0 +1,156 @@
+"""Batched forest prediction benchmark (RGB-D frames of mixed sizes).
+
+    python -m padenti_amd.predict_bench [--train-images 500] [--check-frames 4]
+
+Trains a forest of 5 trees of depth 20 on synthetic RGB-D capsule scenes
+(uint16 x 4: R, G, B, depth mm; 21 classes; 2000 candidates x 20 thresholds),
+then predicts 256 frames (96 x 320x240, 128 x 640x480, 32 x 1280x720 =
+76.2 Mpx) resident in HBM in ONE launch captured in a hipGraph.  Reports
+Mpix/s (graph replay, posteriors + leaf ids of every tree), the p50/p99 of
+per-frame latency (one launch per frame), the HBM roofline of the posterior
+output and, on sampled frames, leaf-id and posterior bit-exactness against
+the CPU oracle.
+"""
+import argparse
+import ctypes
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import _lib as L
+from . import HIPClassifier, HIPTreeTrainer, Tree, synth
+
+FRAMES = [(96, 320, 240), (128, 640, 480), (32, 1280, 720)]
+HBM_PEAK_GBS = 8000.0
+
+
+def train_forest(n_images, n_trees, depth, samples):
+    c = synth.CONFIGS["c4"]
+    ts = synth.training_set("c4", n_images=n_images, samples=samples)
+    p = synth.params_for("c4")
+    tr = HIPTreeTrainer(c["kind"], c["n_classes"])
+    trees = []
+    t0 = time.perf_counter()
+    for t in range(n_trees):
+        tree = Tree(t, depth, c["kind"], c["n_classes"])
+        tr.train(tree, ts, p, 1, depth)
+        trees.append(tree)
+    train_s = time.perf_counter() - t0
+    tr.close()
+    return trees, train_s
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--train-images", type=int, default=500)
+    ap.add_argument("--samples", type=int, default=2000)
+    ap.add_argument("--trees", type=int, default=5)
+    ap.add_argument("--depth", type=int, default=20)
+    ap.add_argument("--reps", type=int, default=10)
+    ap.add_argument("--check-frames", type=int, default=4)
+    ap.add_argument("--scale", type=int, default=1, help="divide the frame counts (debug)")
+    a = ap.parse_args()
+    c = synth.CONFIGS["c4"]
+    nC = c["n_classes"]
+    trees, train_s = train_forest(a.train_images, a.trees, a.depth, a.samples)
+    print(f"[predict] trained {a.trees} trees (depth {a.depth}) in {train_s:.2f} s", file=sys.stderr, flush=True)
+    clf = HIPClassifier(c["kind"], nC)
+    for t in trees:
+        clf << t
+
+    sizes = []
+    for n, w, h in FRAMES:
+        sizes += [(w, h)] * max(1, n // a.scale)
+    frames = []
+    for i, (w, h) in enumerate(sizes):
+        frames.append(synth.gen_image(c["scene"], 7, 2_000_000 + i, w, h, nC)[0])
+    npx = np.array([w * h for w, h in sizes], dtype=np.uint64)
+    total_px = int(npx.sum())
+    img_off = (np.concatenate([[0], np.cumsum(npx)[:-1]]) * 4).astype(np.uint64)
+    post_off = (np.concatenate([[0], np.cumsum(npx)[:-1]]) * nC).astype(np.uint64)
+    leaf_off = (np.concatenate([[0], np.cumsum(npx)[:-1]]) * a.trees).astype(np.uint64)
+    widths = np.array([s[0] for s in sizes], dtype=np.uint32)
+    heights = np.array([s[1] for s in sizes], dtype=np.uint32)
+    lib = L.lib()
+
+    def dalloc(nbytes):
+        p = ctypes.c_void_p()
+        L.check(lib.pd_dev_alloc(0, int(nbytes), ctypes.byref(p)))
+        return p
+
+    host = np.concatenate([f.reshape(-1) for f in frames]).astype(np.uint16)
+    d_pix = dalloc(host.nbytes)
+    d_post = dalloc(total_px * nC * 4)
+    d_leaf = dalloc(total_px * a.trees * 4)
+    L.check(lib.pd_memcpy_h2d(d_pix, L.ptr(host), host.nbytes))
+    ms = ctypes.c_float()
+
+    def run(graph, post=True, leaves=True, frames_sel=None):
+        if frames_sel is None:
+            L.check(lib.pd_forest_predict_frames_dev(clf._h, len(sizes), d_pix, L.ptr(img_off), L.ptr(widths),
+                                                     L.ptr(heights), None, None, d_post if post else None,
+                                                     L.ptr(post_off), d_leaf if leaves else None, L.ptr(leaf_off),
+                                                     graph, ctypes.byref(ms)))
+        else:
+            i = frames_sel
+            io, po, lo = img_off[i:i + 1].copy(), post_off[i:i + 1].copy(), leaf_off[i:i + 1].copy()
+            L.check(lib.pd_forest_predict_frames_dev(clf._h, 1, d_pix, L.ptr(io), L.ptr(widths[i:i + 1].copy()),
+                                                     L.ptr(heights[i:i + 1].copy()), None, None, d_post,
+                                                     L.ptr(po), None, L.ptr(lo), 0, ctypes.byref(ms)))
+        return ms.value
+
+    run(1)   # capture
+    times = [run(1) for _ in range(a.reps)]
+    t_graph = float(np.median(times))
+    run(1, post=True, leaves=False)   # capture the posterior-only graph
+    t_post = float(np.median([run(1, post=True, leaves=False) for _ in range(a.reps)]))
+    lat = []
+    for i in range(len(sizes)):
+        lat.append(run(0, frames_sel=i))
+    lat = np.array(lat)
+    # parity on sampled frames (the largest ones are expensive for the oracle)
+    from oracle import oracle as orc   # checker only
+    run(0)
+    rng = np.random.default_rng(0)
+    check = sorted(set(rng.choice(len(sizes), min(a.check_frames, len(sizes)), replace=False).tolist()))
+    exact = True
+    for i in check:
+        w, h = sizes[i]
+        got = np.zeros(nC * w * h, np.float32)
+        gl_all = np.zeros(a.trees * w * h, np.int32)
+        L.check(lib.pd_memcpy_d2h(L.ptr(got), ctypes.c_void_p(d_post.value + int(post_off[i]) * 4), got.nbytes))
+        L.check(lib.pd_memcpy_d2h(L.ptr(gl_all), ctypes.c_void_p(d_leaf.value + int(leaf_off[i]) * 4),
+                                  gl_all.nbytes))
+        want = orc.predict_posterior(c["kind"], frames[i], None, trees, nC).reshape(-1)
+        exact &= got.tobytes() == want.tobytes()
+        for t in range(a.trees):
+            wl = orc.predict_leaves(c["kind"], frames[i], None, trees[t]).reshape(-1)
+            exact &= bool(np.array_equal(gl_all[t * w * h:(t + 1) * w * h], wl))
+    post_bytes = total_px * (8 + nC * 4)
+    for p in (d_pix, d_post, d_leaf):
+        lib.pd_dev_free(p)
+    out = {
+        "metric": "forest prediction Mpix/s (5 trees, depth 20, posteriors + leaf ids, one hipGraph launch)",
+        "value": total_px / (t_graph * 1e-3) / 1e6,
+        "unit": "Mpix/s",
+        "frames": len(sizes),
+        "pixels": total_px,
+        "ms_graph_launch": t_graph,
+        "ms_posterior_only": t_post,
+        "posterior_only_Mpix_s": total_px / (t_post * 1e-3) / 1e6,
+        "frame_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "roofline": {"bound": "hbm", "achieved": post_bytes / (t_post * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": post_bytes / (t_post * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "train_s": train_s,
+        "parity_frames_checked": check,
+        "bit_exact_vs_oracle": bool(exact),
+        "tree_nodes_split": [int((t.left_children >= 0).sum()) for t in trees],
+    }
+    print(json.dumps(out), flush=True)
+    clf.close()
+
+
+if __name__ == "__main__":
+    main()
