@@ -15,6 +15,11 @@ per launch are
 pixel); achieved = sum B / sum kernel time (HIP events on the trainer's
 stream), peak = 8 TB/s HBM.
 
+lds: the same kernel's LDS-atomic work, one 32-bit LDS add per (active
+sample, candidate feature): U = sum S*nF; achieved = U / kernel time against
+the chip's sustained LDS-atomic rate measured in the same run by the
+library's probe kernel (pd_lds_atomic_bench).
+
 cpu_baseline: the CPU oracle (an op-for-op restatement of the reference
 histogram) timed on this host's cores on a bounded sample of the same
 workload: the root node of the same data set with a subset of the candidate
@@ -70,6 +75,19 @@ def cpu_baseline(ts, params, cfg, target_s):
     }
 
 
+def lds_peak():
+    """Chip-wide LDS atomic adds/s of the probe kernel (best of 3)."""
+    import ctypes
+
+    import padenti_amd as pa
+    ms, rate = ctypes.c_float(), ctypes.c_double()
+    best = 0.0
+    for _ in range(3):
+        pa.check(pa.lib().pd_lds_atomic_bench(0, 4096, 3, ctypes.byref(ms), ctypes.byref(rate)))
+        best = max(best, rate.value)
+    return best
+
+
 def traffic_per_launch(cfg_name):
     """HBM bytes per histogram launch measured with rocprofv3 --pmc FETCH_SIZE /
     WRITE_SIZE on this workload (scaled by the copy-kernel calibration),
@@ -92,6 +110,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--levels-json", default=None, help="write per-level records to this file")
+    ap.add_argument("--per-level", action="store_true", help="print per-level timings and evals/s")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (trees or candidate features "
@@ -133,6 +152,7 @@ def main():
     evals = 0.0
     hist_s = 0.0
     hist_b = 0.0
+    lds_u = 0.0
     launches = 0
     levels = []
     for i in range(args.steps):
@@ -144,13 +164,23 @@ def main():
             launches += s["hist_launches"]
             hist_b += hist_bytes(s["active_samples"], s["frontier_nodes"], params.n_features, params.n_thresholds,
                                  cfg["n_classes"], fd, b_px)
+            lds_u += s["active_samples"] * params.n_features
         levels.append(st)
+        if args.per_level:
+            for s in st:
+                ms = s["hist_ms"] + s["split_ms"] + s["route_ms"]
+                print(f"[bench]   level {s['level']:2d}: nodes {s['frontier_nodes']:7d} samples {s['active_samples']:9d} "
+                      f"slices {s['slices']:3d} hist {s['hist_ms']:8.2f} ms split {s['split_ms']:7.2f} ms "
+                      f"route {s['route_ms']:6.2f} ms  {s['evals'] / max(ms, 1e-6) / 1e9:7.2f} Teval/s",
+                      file=sys.stderr, flush=True)
         print(f"[bench] tree {i % cfg['trees']}: {dt:.3f} s, {e / dt / 1e12:.2f} Teval/s, "
               f"splits {int((tree.left_children >= 0).sum())}", file=sys.stderr, flush=True)
     pa.lib().pd_dev_sync(0)
     total = time.perf_counter() - t_all
     value = evals / total
     achieved = hist_b / hist_s / 1e9 if hist_s > 0 else 0.0
+    lds_p = lds_peak()
+    lds_a = lds_u / hist_s if hist_s > 0 else 0.0
     if args.levels_json:
         with open(args.levels_json, "w") as f:
             json.dump(levels, f, indent=1)
@@ -188,6 +218,7 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic_per_launch(args.config),
         },
+        "lds": {"achieved": lds_a, "peak": lds_p, "unit": "atomic adds/s", "frac": lds_a / lds_p if lds_p else None},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

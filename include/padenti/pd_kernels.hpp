@@ -676,6 +676,27 @@ __global__ __launch_bounds__(256) void k_copy(const uint4 *__restrict__ src, uin
     dst[i] = src[i];
 }
 
+// LDS atomic-add probe: every lane of a 512-thread block adds into its own
+// column of a [64][64] u32 table, rows rotating per step, as the histogram
+// kernel adds into its rank bins (one conflict-free ds_add_u32 per lane per
+// step).  Measures the sustained LDS-atomic rate that bounds the histogram.
+// out: gridDim.x * 512 words (the final table rows, so nothing is dead code).
+__global__ __launch_bounds__(512) void k_lds_add(uint32_t iters, uint32_t *__restrict__ out)
+{
+  __shared__ uint32_t bins[64 * 64];
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  for (uint32_t i = threadIdx.x; i < 64u * 64u; i += blockDim.x) bins[i] = 0u;
+  __syncthreads();
+  uint32_t row = wave * 8u;
+  for (uint32_t it = 0; it < iters; it++) {
+#pragma unroll
+    for (int u = 0; u < 8; u++) atomicAdd(&bins[((row + 5u * (uint32_t)u) & 63u) * 64u + lane], 1u);
+    row += 3u;
+  }
+  __syncthreads();
+  out[(size_t)blockIdx.x * blockDim.x + threadIdx.x] = bins[threadIdx.x];
+}
+
 __global__ void k_log2_det(const float *__restrict__ x, float *__restrict__ y, uint32_t n)
 {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;

@@ -180,6 +180,11 @@ PD_API int pd_forest_predict_frames_dev(pd_forest *fo, uint32_t n_frames, const 
  * `bytes` (read) + `bytes` (write), over `reps` launches. */
 PD_API int pd_copy_bench(int device, uint64_t bytes, int reps, float *ms_per_copy);
 
+/* LDS-atomic probe: 16 blocks of 512 threads per CU, each lane doing
+ * 8 * iters conflict-free 32-bit LDS adds into its own table column (the
+ * histogram kernel's access pattern).  *adds_per_s = chip-wide rate. */
+PD_API int pd_lds_atomic_bench(int device, uint32_t iters, int reps, float *ms_per_launch, double *adds_per_s);
+
 /* ---- device memory helpers (for callers without another allocator) ---- */
 PD_API int pd_dev_alloc(int device, uint64_t bytes, void **out);
 PD_API int pd_dev_free(void *p);

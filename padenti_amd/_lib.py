@@ -22,6 +22,7 @@ PD_MODE_LEVEL_BATCHED = 1
 PD_SYN_BLOBS_F32 = 0
 PD_SYN_CAPSULES_U16 = 1
 PD_SYN_CAPSULES_RGBD = 2
+PD_SYN_BLOBS_U16 = 3
 
 # (image dtype, channels, feature dtype, feature dim) of each compiled hook
 FEATURES = {
@@ -118,6 +119,7 @@ def lib():
         "pd_forest_predict_frames_dev": (I32, [VP, U32, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, I32,
                                                C.POINTER(C.c_float)]),
         "pd_copy_bench": (I32, [I32, U64, I32, C.POINTER(C.c_float)]),
+        "pd_lds_atomic_bench": (I32, [I32, U32, I32, C.POINTER(C.c_float), C.POINTER(C.c_double)]),
         "pd_dev_alloc": (I32, [I32, U64, C.POINTER(VP)]),
         "pd_dev_free": (I32, [VP]),
         "pd_memcpy_h2d": (I32, [VP, VP, U64]),

@@ -288,6 +288,31 @@ int pd_copy_bench(int device, uint64_t bytes, int reps, float *ms_per_copy)
   });
 }
 
+int pd_lds_atomic_bench(int device, uint32_t iters, int reps, float *ms_per_launch, double *adds_per_s)
+{
+  return guarded([&] {
+    require(iters >= 1 && iters <= (1u << 20) && reps >= 1 && ms_per_launch && adds_per_s,
+            "bad LDS-bench arguments");
+    set_device(device);
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    const uint32_t blocks = (uint32_t)prop.multiProcessorCount * 16;
+    DevBuf out;
+    out.ensure((size_t)blocks * 512 * sizeof(uint32_t));
+    hipLaunchKernelGGL(k_lds_add, dim3(blocks), dim3(512), 0, nullptr, iters, out.as<uint32_t>());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+    Event e0, e1;
+    HIPCHK(hipEventRecord(e0.e, nullptr));
+    for (int r = 0; r < reps; r++)
+      hipLaunchKernelGGL(k_lds_add, dim3(blocks), dim3(512), 0, nullptr, iters, out.as<uint32_t>());
+    HIPCHK(hipEventRecord(e1.e, nullptr));
+    HIPCHK(hipEventSynchronize(e1.e));
+    *ms_per_launch = (float)(elapsed_ms(e0, e1) / reps);
+    *adds_per_s = (double)blocks * 512.0 * 8.0 * (double)iters / (*ms_per_launch * 1e-3);
+  });
+}
+
 int pd_dev_alloc(int device, uint64_t bytes, void **out)
 {
   return guarded([&] {

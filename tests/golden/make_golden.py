@@ -53,7 +53,7 @@ def param_table():
 def synth_digests():
     from padenti_amd import synth
     out = {}
-    for cfg, idx in [("c1", 0), ("c1", 5), ("c2", 0), ("c4", 3)]:
+    for cfg, idx in [("c1", 0), ("c1", 5), ("c2", 0), ("c3", 1), ("c4", 3)]:
         c = synth.CONFIGS[cfg]
         px, lab = synth.gen_image(c["scene"], 0, idx, c["w"], c["h"], c["n_classes"])
         smp = synth.gen_samples(0, idx, lab, c["samples"])

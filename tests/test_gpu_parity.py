@@ -238,6 +238,31 @@ def test_train_c2_shape_bit_exact(gpu, chunk):
     tr.close()
 
 
+def test_train_c3_shape_bit_exact(gpu):
+    """C3 shapes (640x480 uint16 blob scenes, 16 classes, 32 thresholds:
+    the largest LDS footprint) on 6 images x 1500 samples, 256 candidates,
+    depth 12, with multi-item nodes (chunk 512) at the top levels."""
+    ts = synth.training_set("c3", n_images=6, samples=1500)
+    p = synth.params_for("c3", n_features=256)
+    tree = pa.Tree(0, 12, pa.PD_FEAT_DEPTH_U16, 16)
+    tr = pa.HIPTreeTrainer(pa.PD_FEAT_DEPTH_U16, 16)
+    tr.set_training_set(ts)
+    tr.set_chunk(512)
+    diag = tr.train(tree, ts, p, 1, 12, diagnostics=True)
+    out = orc.train_tree(orc.Tset(pa.PD_FEAT_DEPTH_U16, ts), p, ts.priors, 0, 12)
+    _check_tree_equal(tree, out, diag)
+    assert int((tree.left_children >= 0).sum()) > 20
+    tr.close()
+
+
+def test_lds_atomic_probe(gpu):
+    ms, rate = ctypes.c_float(), ctypes.c_double()
+    pa.check(pa.lib().pd_lds_atomic_bench(0, 64, 2, ctypes.byref(ms), ctypes.byref(rate)))
+    assert ms.value > 0 and rate.value > 1e11
+    with pytest.raises(pa.PdError):
+        pa.check(pa.lib().pd_lds_atomic_bench(0, 0, 1, ctypes.byref(ms), ctypes.byref(rate)))
+
+
 @pytest.mark.parametrize("kind", [pa.PD_FEAT_DEMO_S16, pa.PD_FEAT_RGBD_U16X4])
 def test_train_other_hooks_bit_exact(gpu, kind):
     if kind == pa.PD_FEAT_DEMO_S16:
