@@ -170,7 +170,7 @@ inline double gain_f64(const uint32_t *left, const uint32_t *tot, uint32_t nC)
 // response exceeds)
 inline int n_thr_bucket(uint32_t nT)
 {
-  static const int buckets[] = {8, 10, 16, 20, 24, 32, 48, 64};
+  static const int buckets[] = {1, 8, 10, 16, 20, 24, 32, 48, 64};
   for (int b : buckets)
     if (nT <= (uint32_t)b) return b;
   return 64;
@@ -204,6 +204,230 @@ inline void check_params(const pd_train_params &P, int fd, bool is_short)
 }
 
 // ---------------------------------------------------------------------------
+// Split-proposal network (host side of pd_mlp.hpp)
+// ---------------------------------------------------------------------------
+struct SplitMix {
+  uint64_t s;
+  explicit SplitMix(uint64_t seed) : s(seed) {}
+  uint64_t next()
+  {
+    uint64_t z = (s += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+  }
+  double uni() { return (double)(next() >> 11) * 0x1p-53; }
+};
+
+struct MlpNet {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  DevBuf theta, m, v, part, gbuf, lbuf;
+  DevBuf W1, W2, W2T, W3, W3T, w4;
+  DevBuf Xb, tb, yb, Z, HT, DZT, dy, lossp, lsteps, perm, Xs, ys;
+  uint32_t ldb = 0;   // batch stride of the training buffers (multiple of SPLIT)
+  uint64_t t_step = 0;
+
+  MlpNet(int dev, uint64_t seed)
+  {
+    device = dev;
+    set_device(device);
+    HIPCHK(hipStreamCreate(&stream));
+    theta.ensure(mlp::NPARAM * sizeof(float));
+    m.ensure(mlp::NPARAM * sizeof(float));
+    v.ensure(mlp::NPARAM * sizeof(float));
+    gbuf.ensure(mlp::NPARAM * sizeof(float));
+    lbuf.ensure(sizeof(float));
+    W1.ensure(mlp::HID * mlp::INP * 2);
+    W2.ensure(mlp::HID * mlp::HID * 2);
+    W2T.ensure(mlp::HID * mlp::HID * 2);
+    W3.ensure(mlp::HID * mlp::HID * 2);
+    W3T.ensure(mlp::HID * mlp::HID * 2);
+    w4.ensure(mlp::HID * 2);
+    init(seed);
+  }
+  ~MlpNet()
+  {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  mlp::Weights weights() const
+  {
+    return mlp::Weights{W1.as<uint16_t>(), W2.as<uint16_t>(),  W3.as<uint16_t>(), W2T.as<uint16_t>(),
+                        W3T.as<uint16_t>(), w4.as<uint16_t>(), theta.as<float>()};
+  }
+  // weights U(-1/sqrt(fan_in), 1/sqrt(fan_in)), biases 0
+  void init(uint64_t seed)
+  {
+    std::vector<float> h(mlp::NPARAM, 0.0f);
+    SplitMix r(seed ^ 0x6d6c7001ull);
+    auto fill = [&](int off, int n, int fan_in) {
+      const double a = 1.0 / std::sqrt((double)fan_in);
+      for (int i = 0; i < n; i++) h[off + i] = (float)((2.0 * r.uni() - 1.0) * a);
+    };
+    fill(mlp::OFF_W1, mlp::HID * mlp::IN, mlp::IN);
+    fill(mlp::OFF_W2, mlp::HID * mlp::HID, mlp::HID);
+    fill(mlp::OFF_W3, mlp::HID * mlp::HID, mlp::HID);
+    fill(mlp::OFF_W4, mlp::HID, mlp::HID);
+    set_params(h.data());
+  }
+  void pack()
+  {
+    hipLaunchKernelGGL(mlp::k_mlp_pack, dim3(mlp::HID * mlp::HID / 256), dim3(256), 0, stream, theta.as<float>(),
+                       W1.as<uint16_t>(), W2.as<uint16_t>(), W2T.as<uint16_t>(), W3.as<uint16_t>(),
+                       W3T.as<uint16_t>(), w4.as<uint16_t>());
+    HIPCHK(hipGetLastError());
+  }
+  // new parameters; the Adam state restarts
+  void set_params(const float *h)
+  {
+    set_device(device);
+    HIPCHK(hipMemcpyAsync(theta.p, h, mlp::NPARAM * sizeof(float), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemsetAsync(m.p, 0, mlp::NPARAM * sizeof(float), stream));
+    HIPCHK(hipMemsetAsync(v.p, 0, mlp::NPARAM * sizeof(float), stream));
+    t_step = 0;
+    pack();
+    HIPCHK(hipStreamSynchronize(stream));
+  }
+  void get_params(float *h)
+  {
+    set_device(device);
+    HIPCHK(hipMemcpyAsync(h, theta.p, mlp::NPARAM * sizeof(float), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+  }
+  void ensure_batch(uint32_t B)
+  {
+    const uint32_t l = (std::max<uint32_t>(B, 1u) + mlp::SPLIT - 1) / mlp::SPLIT * mlp::SPLIT;
+    if (l <= ldb) return;
+    ldb = l;
+    Xb.ensure((size_t)ldb * mlp::INP * 2);
+    tb.ensure((size_t)ldb * 4);
+    yb.ensure((size_t)ldb * 4);
+    Z.ensure((size_t)3 * ldb * mlp::HID * 4);
+    HT.ensure((size_t)4 * mlp::HID * ldb * 2);
+    DZT.ensure((size_t)3 * mlp::HID * ldb * 2);
+    dy.ensure((size_t)ldb * 4);
+    lossp.ensure((size_t)(ldb / mlp::ROWS) * 4);
+    part.ensure((size_t)(ldb / mlp::SPLIT) * mlp::NPARAM * 4);
+  }
+  // inference on device rows (bf16 [n][32]) on stream s
+  void forward(const uint16_t *X, uint64_t n, float *y, hipStream_t s)
+  {
+    const uint64_t blocks = (n + mlp::ROWS - 1) / mlp::ROWS;
+    require(blocks < (1ull << 31), "too many rows");
+    if (!blocks) return;
+    hipLaunchKernelGGL((mlp::k_mlp_fwd<mlp::RowsMem, false>), dim3((uint32_t)blocks), dim3(mlp::THREADS), 0, s,
+                       mlp::RowsMem{X, (uint32_t)n}, (uint32_t)n, weights(), y, (float *)nullptr,
+                       (uint16_t *)nullptr, 0u);
+    HIPCHK(hipGetLastError());
+  }
+  // one step on the n rows staged in Xb / tb: forward, backward, gradient
+  // partials; then Adam (update) and/or the summed gradient into grad_out;
+  // the step's mean squared error into loss_out (device pointers, may be null)
+  void step(uint32_t n, bool update, const pd_adam &opt, float *grad_out, float *loss_out)
+  {
+    require(n >= 1 && n <= ldb, "batch larger than the staged buffers");
+    const uint32_t lrow = (n + mlp::SPLIT - 1) / mlp::SPLIT * mlp::SPLIT;
+    const uint32_t nblk = lrow / mlp::ROWS, ns = lrow / mlp::SPLIT;
+    const mlp::Weights w = weights();
+    uint16_t *ht = HT.as<uint16_t>(), *dz = DZT.as<uint16_t>();
+    const size_t hl = (size_t)mlp::HID * ldb;   // one layer of HT / DZT
+    hipLaunchKernelGGL((mlp::k_mlp_fwd<mlp::RowsMem, true>), dim3(nblk), dim3(mlp::THREADS), 0, stream,
+                       mlp::RowsMem{Xb.as<uint16_t>(), n}, n, w, yb.as<float>(), Z.as<float>(), ht, ldb);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(mlp::k_mlp_bwd, dim3(nblk), dim3(mlp::THREADS), 0, stream, yb.as<float>(), tb.as<float>(), n,
+                       1.0f / (float)n, Z.as<float>(), w, dz, dy.as<float>(), lossp.as<float>(), ldb);
+    HIPCHK(hipGetLastError());
+    if (loss_out) {
+      hipLaunchKernelGGL(mlp::k_mlp_loss, dim3(1), dim3(64), 0, stream, lossp.as<float>(), nblk, 1.0f / (float)n,
+                         loss_out);
+      HIPCHK(hipGetLastError());
+    }
+    float *pt = part.as<float>();
+    hipLaunchKernelGGL(mlp::k_mlp_wgrad, dim3(4, 1, ns), dim3(mlp::THREADS), 0, stream, dz, ht, ldb,
+                       (uint32_t)mlp::IN, pt + mlp::OFF_W1, (size_t)mlp::NPARAM);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(mlp::k_mlp_wgrad, dim3(4, mlp::HID / 32, ns), dim3(mlp::THREADS), 0, stream, dz + hl,
+                       ht + hl, ldb, (uint32_t)mlp::HID, pt + mlp::OFF_W2, (size_t)mlp::NPARAM);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(mlp::k_mlp_wgrad, dim3(4, mlp::HID / 32, ns), dim3(mlp::THREADS), 0, stream, dz + 2 * hl,
+                       ht + 2 * hl, ldb, (uint32_t)mlp::HID, pt + mlp::OFF_W3, (size_t)mlp::NPARAM);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(mlp::k_mlp_bgrad, dim3((4 * mlp::HID + 1 + 255) / 256, ns), dim3(mlp::THREADS), 0, stream,
+                       dz, ht + 3 * hl, dy.as<float>(), ldb, pt, (size_t)mlp::NPARAM);
+    HIPCHK(hipGetLastError());
+    float c1 = 1.0f, c2 = 1.0f;
+    if (update) {
+      t_step++;
+      c1 = (float)(1.0 / (1.0 - std::pow((double)opt.beta1, (double)t_step)));
+      c2 = (float)(1.0 / (1.0 - std::pow((double)opt.beta2, (double)t_step)));
+    }
+    hipLaunchKernelGGL(mlp::k_mlp_adam, dim3((mlp::NPARAM + 255) / 256), dim3(256), 0, stream, theta.as<float>(),
+                       m.as<float>(), v.as<float>(), pt, ns, (size_t)mlp::NPARAM, grad_out, update ? 1 : 0, opt.lr,
+                       opt.beta1, opt.beta2, opt.eps, c1, c2);
+    HIPCHK(hipGetLastError());
+    if (update) pack();
+  }
+  // host rows (FP32 [n][22]) -> bf16 rows [n][32]
+  static void encode(const float *x, uint64_t n, uint16_t *out)
+  {
+    for (uint64_t r = 0; r < n; r++)
+      for (int k = 0; k < mlp::INP; k++) out[r * mlp::INP + k] = k < mlp::IN ? mlp::f2bf(x[r * mlp::IN + k]) : 0;
+  }
+  void stage_host(const float *x, const float *t, uint32_t n)
+  {
+    ensure_batch(n);
+    std::vector<uint16_t> xb((size_t)ldb * mlp::INP, 0);
+    std::vector<float> tv(ldb, 0.0f);
+    encode(x, n, xb.data());
+    if (t) std::copy(t, t + n, tv.begin());
+    HIPCHK(hipMemcpyAsync(Xb.p, xb.data(), xb.size() * 2, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(tb.p, tv.data(), tv.size() * 4, hipMemcpyHostToDevice, stream));
+  }
+  // training loop over device records X (bf16 [N][32]) / t (FP32 [N]):
+  // `epochs` passes, each over a fresh permutation (SplitMix64 Fisher-Yates
+  // of seed, epoch), batches of B rows (the last one shorter)
+  void train(const uint16_t *X, const float *t, uint64_t N, uint32_t B, uint32_t epochs, const pd_adam &opt,
+             uint64_t seed, float *epoch_loss, float *ms)
+  {
+    require(X && t && N >= 1 && N < (1ull << 32) && B >= 1 && B <= (1u << 20) && epochs >= 1,
+            "bad training arguments");
+    set_device(device);
+    ensure_batch(B);
+    const uint64_t steps = (N + B - 1) / B;
+    perm.ensure(N * sizeof(uint32_t));
+    lsteps.ensure(steps * sizeof(float));
+    std::vector<uint32_t> p(N);
+    std::vector<float> ls(steps);
+    Event e0, e1;
+    double total_ms = 0.0;
+    for (uint32_t ep = 0; ep < epochs; ep++) {
+      for (uint64_t i = 0; i < N; i++) p[i] = (uint32_t)i;
+      SplitMix r(seed * 0x9e3779b97f4a7c15ull + ep + 1);
+      for (uint64_t i = N - 1; i > 0; i--) std::swap(p[i], p[r.next() % (i + 1)]);
+      HIPCHK(hipMemcpyAsync(perm.p, p.data(), N * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+      HIPCHK(hipEventRecord(e0.e, stream));
+      for (uint64_t sidx = 0; sidx < steps; sidx++) {
+        const uint64_t base = sidx * B;
+        const uint32_t cnt = (uint32_t)std::min<uint64_t>(B, N - base);
+        hipLaunchKernelGGL(mlp::k_mlp_gather, dim3((ldb * 4 + 255) / 256), dim3(256), 0, stream, X, t,
+                           perm.as<uint32_t>(), base, cnt, ldb, Xb.as<uint16_t>(), tb.as<float>());
+        HIPCHK(hipGetLastError());
+        step(cnt, true, opt, nullptr, lsteps.as<float>() + sidx);
+      }
+      HIPCHK(hipEventRecord(e1.e, stream));
+      HIPCHK(hipMemcpyAsync(ls.data(), lsteps.p, steps * sizeof(float), hipMemcpyDeviceToHost, stream));
+      HIPCHK(hipStreamSynchronize(stream));
+      total_ms += elapsed_ms(e0, e1);
+      double acc = 0.0;
+      for (uint64_t sidx = 0; sidx < steps; sidx++)
+        acc += (double)ls[sidx] * (double)std::min<uint64_t>(B, N - sidx * B);
+      if (epoch_loss) epoch_loss[ep] = (float)(acc / (double)N);
+    }
+    if (ms) *ms = (float)total_ms;
+  }
+};
+
+// ---------------------------------------------------------------------------
 // Trainer
 // ---------------------------------------------------------------------------
 struct TrainerBase {
@@ -212,6 +436,16 @@ struct TrainerBase {
   uint64_t hist_budget = 8ull << 30;
   uint32_t chunk = 0;
   hipStream_t stream = nullptr;
+  // record sink of the plain trainer (device rows bf16 [cap][32], gains [cap])
+  uint16_t *sink_x = nullptr;
+  float *sink_t = nullptr;
+  float *sink_rel = nullptr;
+  uint64_t sink_cap = 0, sink_n = 0;
+  uint32_t sink_R = 0;
+  // split proposal (PD_MODE_PROPOSAL)
+  MlpNet *prop_mlp = nullptr;
+  uint32_t prop_keep = 0, prop_screen = 0, prop_min = 0;
+  int prop_judge = 0;
   virtual ~TrainerBase()
   {
     if (stream) (void)hipStreamDestroy(stream);
@@ -253,6 +487,10 @@ struct Trainer final : TrainerBase {
   uint32_t cu_count = 256;
   uint32_t n_xcc = 1;
   size_t lds_limit = 65536;
+  // candidate map of the histogram launch in flight (proposal slices)
+  const uint32_t *d_map_cur = nullptr;
+  DevBuf d_gains, d_emit_slots, d_emit_base, d_emit_best, d_emit_in, d_emit_node, d_pslot_in, d_pslot_node, d_scores,
+      d_map;
 
   Trainer(int dev, uint32_t ncl)
   {
@@ -372,7 +610,8 @@ struct Trainer final : TrainerBase {
     hipLaunchKernelGGL((k_hist<F, NTP>), dim3(n_items), dim3(HIST_THREADS), lds, stream, d_pixels.as<I>(),
                        d_imgs.as<ImgDesc>(), act, d_items.as<HistItem>(), item_base, d_slot_node.as<int32_t>(),
                        d_slot_mode.as<uint8_t>(), d_slot_hidx.as<int32_t>(), d_tot.as<uint32_t>(), bnd, tree, nF,
-                       nT, nC, d_H.as<uint32_t>(), d_cand_key.as<unsigned long long>(), d_cand_left.as<uint32_t>());
+                       nT, nC, d_H.as<uint32_t>(), d_cand_key.as<unsigned long long>(), d_cand_left.as<uint32_t>(),
+                       d_map_cur);
     HIPCHK(hipGetLastError());
   }
 
@@ -381,6 +620,7 @@ struct Trainer final : TrainerBase {
   {
     if (!n_items) return;
     switch (n_thr_bucket(nT)) {
+      case 1: launch_hist_t<1>(n_items, item_base, act, bnd, tree, nF, nT); break;
       case 8: launch_hist_t<8>(n_items, item_base, act, bnd, tree, nF, nT); break;
       case 10: launch_hist_t<10>(n_items, item_base, act, bnd, tree, nF, nT); break;
       case 16: launch_hist_t<16>(n_items, item_base, act, bnd, tree, nF, nT); break;
@@ -540,6 +780,212 @@ struct Trainer final : TrainerBase {
     HIPCHK(hipStreamSynchronize(stream));
   }
 
+  // network input of a node: class histogram n_c / n (16) and depth / 16
+  void slot_features(uint32_t node, uint32_t level, const std::vector<uint32_t> &classTot, float *out) const
+  {
+    uint32_t N = 0;
+    for (uint32_t c = 0; c < nC; c++) N += classTot[(size_t)node * nC + c];
+    for (uint32_t c = 0; c < (uint32_t)mlp::HIST_IN; c++)
+      out[c] = (c < nC && N) ? (float)classTot[(size_t)node * nC + c] / (float)N : 0.0f;
+    out[16] = (float)level / 16.0f;
+  }
+  static mlp::RowNorm make_norm(const Bounds<F> &bnd)
+  {
+    mlp::RowNorm nm{};
+    for (int j = 0; j < 4; j++) {
+      float b = 0.0f;
+      if (j < FD) b = std::max(std::fabs((float)bnd.lo[j]), std::fabs((float)bnd.hi[j]));
+      nm.inv_p[j] = b > 0.0f ? 1.0f / b : 0.0f;
+    }
+    const float range = (float)bnd.thi - (float)bnd.tlo;
+    nm.thr_lo = (float)bnd.tlo;
+    nm.thr_scale = range > 0.0f ? 2.0f / range : 0.0f;
+    return nm;
+  }
+
+  // Records of the slots [s0, s0 + ns) of `sub` whose H blocks and all pair
+  // gains (gains[q][t * nFe + f]) were just computed.
+  void emit_records(const std::vector<SlotInfo> &sub, size_t s0, uint32_t ns, uint32_t nFe, uint32_t nT,
+                    const Bounds<F> &bnd, uint32_t tree, const std::vector<uint32_t> &classTot, uint32_t level,
+                    const float *gains, const std::vector<uint32_t> &res)
+  {
+    const uint32_t R = std::min<uint32_t>(sink_R, nT * nFe);
+    if (!R) return;
+    std::vector<uint32_t> es;
+    std::vector<uint64_t> eb;
+    std::vector<float> ebest;
+    std::vector<float> sin((size_t)ns * 17, 0.0f);
+    std::vector<int32_t> snode(ns);
+    for (uint32_t q = 0; q < ns; q++) {
+      const SlotInfo &si = sub[s0 + q];
+      snode[q] = si.node;
+      slot_features((uint32_t)si.node, level, classTot, &sin[(size_t)q * 17]);
+      if (si.trainable && si.count > 0 && sink_n + R <= sink_cap) {
+        es.push_back(q);
+        eb.push_back(sink_n);
+        float g;
+        std::memcpy(&g, &res[(size_t)q * (nC + 2) + 1], 4);
+        ebest.push_back(g);
+        sink_n += R;
+      }
+    }
+    if (es.empty()) return;
+    d_emit_in.ensure(sin.size() * sizeof(float));
+    d_emit_node.ensure(ns * sizeof(int32_t));
+    d_emit_slots.ensure(es.size() * sizeof(uint32_t));
+    d_emit_base.ensure(eb.size() * sizeof(uint64_t));
+    d_emit_best.ensure(ebest.size() * sizeof(float));
+    HIPCHK(hipMemcpyAsync(d_emit_best.p, ebest.data(), ebest.size() * sizeof(float), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(d_emit_in.p, sin.data(), sin.size() * sizeof(float), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(d_emit_node.p, snode.data(), ns * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(d_emit_slots.p, es.data(), es.size() * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(d_emit_base.p, eb.data(), eb.size() * sizeof(uint64_t), hipMemcpyHostToDevice, stream));
+    const uint64_t threads = (uint64_t)es.size() * R;
+    hipLaunchKernelGGL((k_emit<F>), dim3((uint32_t)((threads + 255) / 256)), dim3(256), 0, stream, gains,
+                       d_emit_slots.as<uint32_t>(), d_emit_base.as<uint64_t>(), d_emit_best.as<float>(),
+                       (uint32_t)es.size(), R, d_emit_in.as<float>(), d_emit_node.as<int32_t>(), nFe, nT, tree, bnd,
+                       make_norm(bnd), sink_x, sink_t, sink_rel);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(stream));
+  }
+
+  // Histogram + split search of the frontier slots `idx` (indices into
+  // `slots`) with nFe candidates each: the sampler's candidates 0..nFe-1, or,
+  // with a candidate map (d_map [idx.size()][nFe] on the device, h_map its
+  // host copy), the listed sampler indices.  Results go to res_all at each
+  // slot's index with k in sampler terms (t * P.n_features + f).  With
+  // `emit`, every slot gets its H block and the FP32 gains of all pairs, and
+  // the record sink receives samples of them.
+  void evaluate(const std::vector<SlotInfo> &slots, const std::vector<uint32_t> &idx, const uint4 *act,
+                const pd_train_params &P, uint32_t nFe, const uint32_t *d_map_all,
+                const std::vector<uint32_t> *h_map, const Bounds<F> &bnd, uint32_t tree, int mode, uint32_t ch,
+                const std::vector<uint32_t> &classTot, uint32_t level, bool emit, std::vector<uint32_t> &res_all,
+                pd_level_stats &st)
+  {
+    if (idx.empty()) return;
+    const uint32_t nT = P.n_thresholds, nFp = P.n_features;
+    pd_train_params Pe = P;
+    Pe.n_features = nFe;
+    std::vector<SlotInfo> sub(idx.size());
+    for (size_t i = 0; i < idx.size(); i++) sub[i] = slots[idx[i]];
+    const size_t per_bytes = (size_t)nC * nT * nFe * sizeof(uint32_t);
+    size_t h_per_slice = std::max<size_t>(1, hist_budget / per_bytes);   // H blocks per slice
+    if (emit)   // the all-pair gains of a slice stay within 1 GiB
+      h_per_slice = std::min<size_t>(h_per_slice, std::max<size_t>(1, (1ull << 30) / ((size_t)nT * nFe * 4)));
+    Event e0, e1, e2;
+    for (size_t s0 = 0; s0 < sub.size();) {
+      // a slice holds at most h_per_slice slots that need an H block
+      size_t s1 = s0, nh = 0;
+      while (s1 < sub.size()) {
+        const uint32_t cnt = sub[s1].trainable ? sub[s1].count : 0u;
+        const bool needs_h = emit || cnt > ch;
+        if (needs_h && nh == h_per_slice) break;
+        nh += needs_h ? 1 : 0;
+        s1++;
+      }
+      const uint32_t ns = (uint32_t)(s1 - s0);
+      st.slices++;
+      std::vector<uint32_t> tot((size_t)ns * nC);
+      for (uint32_t q = 0; q < ns; q++)
+        std::copy(&classTot[(size_t)sub[s0 + q].node * nC], &classTot[(size_t)sub[s0 + q].node * nC] + nC,
+                  &tot[(size_t)q * nC]);
+      std::vector<uint32_t> hslots, res;
+      d_map_cur = d_map_all ? d_map_all + s0 * nFe : nullptr;
+      st.hist_launches += hist_slice(sub, s0, s1, act, Pe, bnd, tree, mode, ch, tot, emit, hslots, &e0, &e1);
+      d_map_cur = nullptr;
+      float *gains = nullptr;
+      if (emit) {
+        d_gains.ensure((size_t)ns * nT * nFe * sizeof(float));
+        gains = d_gains.as<float>();
+      }
+      split_slice(ns, hslots, nT, nFe, mode, res, gains);
+      HIPCHK(hipEventRecord(e2.e, stream));
+      HIPCHK(hipEventSynchronize(e2.e));
+      st.hist_ms += elapsed_ms(e0, e1);
+      st.split_ms += elapsed_ms(e1, e2);
+      if (emit) emit_records(sub, s0, ns, nFe, nT, bnd, tree, classTot, level, gains, res);
+      for (uint32_t q = 0; q < ns; q++) {
+        const uint32_t *r = &res[(size_t)q * (nC + 2)];
+        const uint32_t t = r[0] / nFe, f = r[0] % nFe;
+        const uint32_t fs = h_map ? (*h_map)[(s0 + q) * nFe + f] : f;
+        uint32_t *o = &res_all[(size_t)idx[s0 + q] * (nC + 2)];
+        o[0] = t * nFp + fs;
+        std::copy(r + 1, r + nC + 2, o + 1);
+      }
+      s0 = s1;
+    }
+  }
+
+  // Split proposal of the slots idx: the network scores the pool of
+  // K = P.n_features candidates of every node (one threshold each), the
+  // best `prop_keep` are histogrammed exactly; optionally the exhaustive
+  // evaluation of the whole pool is run beside it as the judge.
+  void propose(const std::vector<SlotInfo> &slots, const std::vector<uint32_t> &idx, const uint4 *act,
+               const pd_train_params &P, const Bounds<F> &bnd, uint32_t tree, const std::vector<uint32_t> &classTot,
+               uint32_t level, std::vector<uint32_t> &res_all, pd_level_stats &st)
+  {
+    const uint32_t K = P.n_features, keep = prop_keep;
+    const mlp::RowNorm nm = make_norm(bnd);
+    const size_t g_max = std::max<size_t>(1, std::min<size_t>(idx.size(), (512ull << 20) / ((size_t)K * 4)));
+    Event ea, eb, ec;
+    for (size_t g0 = 0; g0 < idx.size(); g0 += g_max) {
+      const size_t G = std::min(g_max, idx.size() - g0);
+      const std::vector<uint32_t> gidx(idx.begin() + g0, idx.begin() + g0 + G);
+      std::vector<float> sin(G * 17);
+      std::vector<int32_t> snode(G);
+      uint64_t gact = 0;
+      for (size_t i = 0; i < G; i++) {
+        const SlotInfo &si = slots[gidx[i]];
+        snode[i] = si.node;
+        slot_features((uint32_t)si.node, level, classTot, &sin[i * 17]);
+        gact += si.count;
+      }
+      const uint64_t rows = (uint64_t)G * K;
+      require(rows < (1ull << 32), "proposal pool too large");
+      d_pslot_in.ensure(sin.size() * sizeof(float));
+      d_pslot_node.ensure(G * sizeof(int32_t));
+      d_scores.ensure(rows * sizeof(float));
+      d_map.ensure(G * keep * sizeof(uint32_t));
+      HIPCHK(hipMemcpyAsync(d_pslot_in.p, sin.data(), sin.size() * sizeof(float), hipMemcpyHostToDevice, stream));
+      HIPCHK(hipMemcpyAsync(d_pslot_node.p, snode.data(), G * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+      const PoolRows<F> src{d_pslot_in.as<float>(), d_pslot_node.as<int32_t>(), K, tree, (uint32_t)rows, bnd, nm};
+      HIPCHK(hipEventRecord(ea.e, stream));
+      hipLaunchKernelGGL((mlp::k_mlp_fwd<PoolRows<F>, false>), dim3((uint32_t)((rows + mlp::ROWS - 1) / mlp::ROWS)),
+                         dim3(mlp::THREADS), 0, stream, src, (uint32_t)rows, prop_mlp->weights(),
+                         d_scores.as<float>(), (float *)nullptr, (uint16_t *)nullptr, 0u);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipEventRecord(eb.e, stream));
+      const uint32_t screen = std::min<uint32_t>(K, std::max<uint32_t>(keep, prop_screen));
+      hipLaunchKernelGGL(mlp::k_topk, dim3((uint32_t)G), dim3(256), 0, stream, d_scores.as<float>(), K, screen, keep,
+                         d_map.as<uint32_t>());
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipEventRecord(ec.e, stream));
+      std::vector<uint32_t> hmap(G * keep);
+      HIPCHK(hipMemcpyAsync(hmap.data(), d_map.p, hmap.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+      HIPCHK(hipStreamSynchronize(stream));
+      st.mlp_ms += elapsed_ms(ea, eb);
+      st.select_ms += elapsed_ms(eb, ec);
+      evaluate(slots, gidx, act, P, keep, d_map.as<uint32_t>(), &hmap, bnd, tree, PD_MODE_LEVEL_BATCHED,
+               pick_chunk(gact, (keep + 63) / 64), classTot, level, false, res_all, st);
+      if (prop_judge) {
+        std::vector<uint32_t> res_j(slots.size() * (nC + 2), 0u);
+        pd_level_stats js{};
+        evaluate(slots, gidx, act, P, K, nullptr, nullptr, bnd, tree, PD_MODE_LEVEL_BATCHED,
+                 pick_chunk(gact, (K + 63) / 64), classTot, level, false, res_j, js);
+        st.judge_ms += js.hist_ms + js.split_ms;
+        for (uint32_t q : gidx) {
+          float ge, gp;
+          std::memcpy(&ge, &res_j[(size_t)q * (nC + 2) + 1], 4);
+          std::memcpy(&gp, &res_all[(size_t)q * (nC + 2) + 1], 4);
+          st.judged_nodes++;
+          st.gain_exhaustive += ge;
+          st.gain_proposal += gp;
+          if (ge > 0.0f) st.gain_loss_rel += ((double)ge - (double)gp) / (double)ge;
+        }
+      }
+    }
+  }
+
   void train(const pd_train_params &P, uint32_t tree_id, uint32_t depth, uint32_t end_depth, int mode,
              const float *prior, int32_t *left, void *feats_v, void *thrs_v, float *post, uint32_t *node_k,
              float *node_gain, double *node_gain64, pd_level_stats *ls) override
@@ -548,7 +994,14 @@ struct Trainer final : TrainerBase {
     check_params(P, FD, sizeof(T) == 2);
     require(depth >= 1 && depth <= 24, "tree depth must be in [1, 24]");
     require(end_depth >= 1 && end_depth <= depth, "end_depth must be in [1, tree depth]");
-    require(mode == PD_MODE_PER_NODE || mode == PD_MODE_LEVEL_BATCHED, "unknown training mode");
+    require(mode == PD_MODE_PER_NODE || mode == PD_MODE_LEVEL_BATCHED || mode == PD_MODE_PROPOSAL,
+            "unknown training mode");
+    if (mode == PD_MODE_PROPOSAL) {
+      require(prop_mlp != nullptr, "PD_MODE_PROPOSAL needs pd_trainer_set_proposal first");
+      require(prop_keep >= 1 && prop_keep <= P.n_features, "proposal keep must be in [1, n_features]");
+      require(nC <= (uint32_t)mlp::HIST_IN, "the proposal network takes at most 16 classes");
+      require(prop_mlp->device == device, "proposal network on another device");
+    }
     require(nC <= 64, "at most 64 classes");
     require(prior && left && feats_v && thrs_v && post, "null tree array");
     set_device(device);
@@ -581,10 +1034,7 @@ struct Trainer final : TrainerBase {
     uint64_t n_act = S;
 
     std::vector<SlotInfo> slots{SlotInfo{0, 0u, (uint32_t)S, S > thr}};
-    Event e0, e1, e2, e3, e4;
-    const size_t per = (size_t)nC * nT * nF;
-    const size_t per_bytes = per * sizeof(uint32_t);
-    const size_t h_per_slice = std::max<size_t>(1, hist_budget / per_bytes);   // H blocks per slice
+    Event e3, e4;
 
     for (uint32_t d = 1; d < end_depth; d++) {
       const auto t_start = std::chrono::steady_clock::now();
@@ -602,67 +1052,57 @@ struct Trainer final : TrainerBase {
       st.evals = (double)active * nF * nT;
       uint32_t ch = pick_chunk(active, (nF + 63) / 64);
       std::vector<uint8_t> split_flag(slots.size(), 0);
-      for (size_t s0 = 0; s0 < slots.size();) {
-        // a slice holds at most h_per_slice multi-chunk slots (the only ones
-        // that need an H block)
-        size_t s1 = s0, nh = 0;
-        while (s1 < slots.size()) {
-          const uint32_t cnt = slots[s1].trainable ? slots[s1].count : 0u;
-          const bool needs_h = cnt > ch;
-          if (needs_h && nh == h_per_slice) break;
-          nh += needs_h ? 1 : 0;
-          s1++;
+      // per-slot result: k = t * nF + f (sampler index f), FP32 gain bits, left counts
+      std::vector<uint32_t> res_all(slots.size() * (nC + 2), 0u);
+      std::vector<uint32_t> plain_idx, prop_idx;
+      for (uint32_t q = 0; q < (uint32_t)slots.size(); q++) {
+        const bool pr = mode == PD_MODE_PROPOSAL && slots[q].trainable && slots[q].count >= prop_min;
+        (pr ? prop_idx : plain_idx).push_back(q);
+      }
+      const int lmode = mode == PD_MODE_PER_NODE ? PD_MODE_PER_NODE : PD_MODE_LEVEL_BATCHED;
+      evaluate(slots, plain_idx, act, P, nF, nullptr, nullptr, bnd, tree_id, lmode, ch, classTot, d,
+               sink_x != nullptr && mode != PD_MODE_PROPOSAL, res_all, st);
+      if (!prop_idx.empty()) propose(slots, prop_idx, act, P, bnd, tree_id, classTot, d, res_all, st);
+      if (mode == PD_MODE_PROPOSAL) {
+        for (uint32_t q : plain_idx)
+          if (slots[q].trainable) st.fallback_nodes++;
+        st.proposal_nodes += (uint32_t)prop_idx.size();
+      }
+      // node update, cl_tree_trainer_impl_learn_best_featthr.hpp:101-233
+      for (uint32_t q = 0; q < (uint32_t)slots.size(); q++) {
+        const uint32_t n = (uint32_t)slots[q].node;
+        const uint32_t *r = &res_all[(size_t)q * (nC + 2)];
+        const uint32_t k = r[0];
+        float g;
+        std::memcpy(&g, &r[1], 4);
+        const uint32_t *lh = r + 2;
+        const uint32_t *tt = &classTot[(size_t)n * nC];
+        if (node_k) node_k[n] = k;
+        if (node_gain) node_gain[n] = g;
+        if (node_gain64) node_gain64[n] = gain_f64(lh, tt, nC);
+        uint32_t lS = 0, rS = 0;
+        for (uint32_t c = 0; c < nC; c++) {
+          lS += lh[c];
+          rS += tt[c] - lh[c];
         }
-        const uint32_t ns = (uint32_t)(s1 - s0);
-        st.slices++;
-        std::vector<uint32_t> tot((size_t)ns * nC);
-        for (uint32_t q = 0; q < ns; q++)
-          std::copy(&classTot[(size_t)slots[s0 + q].node * nC], &classTot[(size_t)slots[s0 + q].node * nC] + nC,
-                    &tot[(size_t)q * nC]);
-        std::vector<uint32_t> hslots, res;
-        st.hist_launches += hist_slice(slots, s0, s1, act, P, bnd, tree_id, mode, ch, tot, false, hslots, &e0, &e1);
-        split_slice(ns, hslots, nT, nF, mode, res);
-        HIPCHK(hipEventRecord(e2.e, stream));
-        HIPCHK(hipEventSynchronize(e2.e));
-        st.hist_ms += elapsed_ms(e0, e1);
-        st.split_ms += elapsed_ms(e1, e2);
-        // node update, cl_tree_trainer_impl_learn_best_featthr.hpp:101-233
-        for (uint32_t q = 0; q < ns; q++) {
-          const uint32_t n = (uint32_t)slots[s0 + q].node;
-          const uint32_t *r = &res[(size_t)q * (nC + 2)];
-          const uint32_t k = r[0];
-          float g;
-          std::memcpy(&g, &r[1], 4);
-          const uint32_t *lh = r + 2;
-          const uint32_t *tt = &classTot[(size_t)n * nC];
-          if (node_k) node_k[n] = k;
-          if (node_gain) node_gain[n] = g;
-          if (node_gain64) node_gain64[n] = gain_f64(lh, tt, nC);
-          uint32_t lS = 0, rS = 0;
-          for (uint32_t c = 0; c < nC; c++) {
-            lS += lh[c];
-            rS += tt[c] - lh[c];
-          }
-          if (!lS || !rS) continue;
-          const uint32_t bt = k / nF, bf = k % nF;
-          const size_t L = 2 * (size_t)n + 1, R = L + 1;
-          nodeTot[L] = lS;
-          nodeTot[R] = rS;
-          for (uint32_t c = 0; c < nC; c++) {
-            classTot[L * nC + c] = lh[c];
-            classTot[R * nC + c] = tt[c] - lh[c];
-            post[L * nC + c] = (float)lh[c] / (float)lS;
-            post[R * nC + c] = (float)(tt[c] - lh[c]) / (float)rS;
-          }
-          left[L] = -1;
-          left[R] = -1;
-          candidate_params<T, FD>(tree_id, n, bf, bnd.lo, bnd.hi, &feats[(size_t)n * FD]);
-          thrs[n] = candidate_threshold<T>(tree_id, n, bf, bt, bnd.tlo, bnd.thi);
-          left[n] = (int32_t)L;
-          split_flag[s0 + q] = 1;
-          st.split_nodes++;
+        if (!lS || !rS) continue;
+        const uint32_t bt = k / nF, bf = k % nF;
+        const size_t L = 2 * (size_t)n + 1, R = L + 1;
+        nodeTot[L] = lS;
+        nodeTot[R] = rS;
+        for (uint32_t c = 0; c < nC; c++) {
+          classTot[L * nC + c] = lh[c];
+          classTot[R * nC + c] = tt[c] - lh[c];
+          post[L * nC + c] = (float)lh[c] / (float)lS;
+          post[R * nC + c] = (float)(tt[c] - lh[c]) / (float)rS;
         }
-        s0 = s1;
+        left[L] = -1;
+        left[R] = -1;
+        candidate_params<T, FD>(tree_id, n, bf, bnd.lo, bnd.hi, &feats[(size_t)n * FD]);
+        thrs[n] = candidate_threshold<T>(tree_id, n, bf, bt, bnd.tlo, bnd.thi);
+        left[n] = (int32_t)L;
+        split_flag[q] = 1;
+        st.split_nodes++;
       }
       if (d + 1 >= end_depth) {
         st.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
@@ -1099,6 +1539,9 @@ void feature_eval_t(const void *image, uint32_t w, uint32_t h, uint32_t n, const
 // Opaque handles of the C ABI.
 struct pd_trainer {
   std::unique_ptr<pd::engine::TrainerBase> impl;
+};
+struct pd_mlp {
+  std::unique_ptr<pd::engine::MlpNet> impl;
 };
 struct pd_forest {
   std::unique_ptr<pd::engine::ForestBase> impl;

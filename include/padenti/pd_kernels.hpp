@@ -27,6 +27,7 @@
 #include <stdint.h>
 #include "pd_math.hpp"
 #include "pd_features.hpp"
+#include "pd_mlp.hpp"
 
 #pragma clang fp contract(off)
 
@@ -67,7 +68,7 @@ constexpr int rank_words() { return (NTP + 2) / 2; }
 
 // power-of-two slot count of the per-lane sorted threshold list
 template <int NTP>
-constexpr int sorted_slots() { return NTP <= 8 ? 8 : (NTP <= 16 ? 16 : (NTP <= 32 ? 32 : 64)); }
+constexpr int sorted_slots() { return NTP <= 1 ? 1 : (NTP <= 8 ? 8 : (NTP <= 16 ? 16 : (NTP <= 32 ? 32 : 64))); }
 
 template <class F, int NTP>
 constexpr size_t hist_lds_bytes(uint32_t nC)
@@ -151,7 +152,7 @@ __global__ __launch_bounds__(HIST_THREADS) void k_hist(
     const int32_t *__restrict__ slot_node, const uint8_t *__restrict__ slot_mode,
     const int32_t *__restrict__ slot_hidx, const uint32_t *__restrict__ tot, Bounds<F> bnd, uint32_t tree,
     uint32_t nF, uint32_t nT, uint32_t nC, uint32_t *__restrict__ H, unsigned long long *__restrict__ cand_key,
-    uint32_t *__restrict__ cand_left)
+    uint32_t *__restrict__ cand_left, const uint32_t *__restrict__ cand_map)
 {
   using T = typename F::FeatType;
   constexpr int FD = F::FeatDim;
@@ -173,17 +174,20 @@ __global__ __launch_bounds__(HIST_THREADS) void k_hist(
   const uint32_t f = it.ftile * 64u + lane;
   const bool fv = f < nF;
   const uint32_t node = (uint32_t)slot_node[it.slot];
+  // sampler index of the lane's candidate: f itself, or (proposal) the
+  // f-th entry of the slot's selected candidate list
+  const uint32_t fs = (cand_map && fv) ? cand_map[(size_t)it.slot * nF + f] : f;
 
   for (uint32_t i = tid; i < nC * RW * 64u; i += HIST_THREADS) bins[i] = 0u;
 
   // candidate parameters (wave 0) and thresholds (wave 1), one MD5 chain each
   if (tid < 64) {
     T p[FD];
-    candidate_params<T, FD>(tree, node, f, bnd.lo, bnd.hi, p);
+    candidate_params<T, FD>(tree, node, fs, bnd.lo, bnd.hi, p);
 #pragma unroll
     for (int j = 0; j < FD; j++) sp[j * 64 + lane] = p[j];
   } else if (tid < 128) {
-    U4 s = {tree, node, f, 1u};
+    U4 s = {tree, node, fs, 1u};
 #pragma unroll
     for (int t = 0; t < NTP; t++) {
       if ((t & 3) == 0 && (uint32_t)t < nT) s = md5rand(s);
@@ -313,6 +317,87 @@ __global__ __launch_bounds__(HIST_THREADS) void k_hist(
     const uint32_t l = sl[(k / nF) * 64 + wl];
     cand_left[ci * nC + tid] = prefix_at(bins, RW, tid, l, wl);
   }
+}
+
+// ---------------------------------------------------------------------------
+// Split proposal: input rows of the network.
+// slot_in[q][17] = the node's class histogram n_c / n (16, zero padded) and
+// its depth feature; candidate c of node q is the sampler's candidate c
+// (parameters from seed (tree, node, c, 0), the first threshold of seed
+// (tree, node, c, 1)), so the pool of K candidates is exactly the plain
+// trainer's candidate set with nF = K, nT = 1.
+// ---------------------------------------------------------------------------
+template <class F>
+__device__ __forceinline__ uint4 candidate_row(const float *slot_in, uint32_t q4, uint32_t tree, uint32_t node,
+                                               uint32_t cand, uint32_t t, const Bounds<F> &bnd,
+                                               const mlp::RowNorm &nm)
+{
+  using T = typename F::FeatType;
+  float v[8];
+  if (q4 < 2) {
+#pragma unroll
+    for (int j = 0; j < 8; j++) v[j] = slot_in[q4 * 8 + j];
+  } else if (q4 == 2) {
+    T p[F::FeatDim];
+    candidate_params<T, F::FeatDim>(tree, node, cand, bnd.lo, bnd.hi, p);
+    const T th = candidate_threshold<T>(tree, node, cand, t, bnd.tlo, bnd.thi);
+    float pf[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) pf[j] = j < F::FeatDim ? (float)p[j] : 0.0f;
+    mlp::cand_features(nm, pf, (float)th, v, slot_in[16]);
+    v[6] = 0.0f;
+    v[7] = 0.0f;
+  } else {
+#pragma unroll
+    for (int j = 0; j < 8; j++) v[j] = 0.0f;
+  }
+  return mlp::pack8(v);
+}
+
+template <class F>
+struct PoolRows {
+  const float *slot_in;       // [n_slots][17]
+  const int32_t *slot_node;   // [n_slots]
+  uint32_t K, tree, n_rows;
+  Bounds<F> bnd;
+  mlp::RowNorm nm;
+  __device__ __forceinline__ uint4 fill(uint32_t row, uint32_t q4) const
+  {
+    if (row >= n_rows) return make_uint4(0u, 0u, 0u, 0u);
+    const uint32_t q = row / K, cand = row % K;
+    return candidate_row<F>(slot_in + (size_t)q * 17, q4, tree, (uint32_t)slot_node[q], cand, 0u, bnd, nm);
+  }
+};
+
+// Record emitter of the plain trainer: for emitting slot e (slice-local slot
+// emit_slots[e]), R records (input row, measured FP32 gain) of pairs
+// k = t*nF + f drawn by MD5 of (tree, node, j, 7) (all pairs in order when
+// R >= nT*nF), written at out_base[e] + j; out_rel (optional) receives the
+// gain relative to the node's best gain over all pairs, best[e].
+template <class F>
+__global__ void k_emit(const float *__restrict__ gains, const uint32_t *__restrict__ emit_slots,
+                       const uint64_t *__restrict__ out_base, const float *__restrict__ best, uint32_t ne, uint32_t R,
+                       const float *__restrict__ slot_in, const int32_t *__restrict__ slot_node, uint32_t nF,
+                       uint32_t nT, uint32_t tree, Bounds<F> bnd, mlp::RowNorm nm, uint16_t *__restrict__ out_x,
+                       float *__restrict__ out_t, float *__restrict__ out_rel)
+{
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t e = (uint32_t)(i / R), j = (uint32_t)(i % R);
+  if (e >= ne) return;
+  const uint32_t q = emit_slots[e];
+  const uint32_t node = (uint32_t)slot_node[q];
+  const uint32_t npairs = nF * nT;
+  uint32_t k = j;
+  if (R < npairs) k = md5rand(U4{tree, node, j, 7u}).x % npairs;
+  const uint32_t f = k % nF, t = k / nF;
+  const uint64_t o = out_base[e] + j;
+#pragma unroll
+  for (uint32_t q4 = 0; q4 < 4; q4++)
+    *(uint4 *)(out_x + o * mlp::INP + q4 * 8) = candidate_row<F>(slot_in + (size_t)q * 17, q4, tree, node, f, t,
+                                                                 bnd, nm);
+  const float g = gains[(size_t)q * npairs + k];
+  out_t[o] = g;
+  if (out_rel) out_rel[o] = best[e] > 0.0f ? g / best[e] : 0.0f;
 }
 
 // Zero the H region of every slot that several chunks add into.

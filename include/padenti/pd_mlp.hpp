@@ -1,0 +1,536 @@
+// Split-proposal network: 22 inputs (normalised node class histogram [16],
+// node depth, candidate u, v and threshold) -> 3 hidden layers of 256 units
+// (GELU, tanh form) -> predicted information gain of the candidate.
+//
+// Numerics: BF16 weights and activations, FP32 accumulation on the gfx950
+// matrix cores (v_mfma_f32_16x16x32_bf16), FP32 bias / GELU / output layer,
+// FP32 master weights and Adam state.  Gradients of the hidden layers are
+// carried in BF16 between the matrix products, as the activations are.
+//
+// Kernels (one 64-row tile per 256-thread block, 4 wave64s):
+//   k_mlp_fwd     all layers fused; the tile's activations stay in LDS.  A
+//                 wave owns 64 output columns: 4 x 4 MFMA 16x16 tiles, the
+//                 A operand (activations) from LDS, the B operand (weights,
+//                 [out][in] row-major) straight from L2.  Training mode also
+//                 stores the FP32 pre-activations Z and the transposed BF16
+//                 activations H^T (the weight gradient reads them as
+//                 contiguous MFMA fragments).
+//   k_mlp_bwd     dL/dy, then dZ3 -> dZ2 -> dZ1 with the transposed weights,
+//                 dZ^T stored in BF16.
+//   k_mlp_wgrad   dW = dZ^T H over one 512-row split of the batch, per 64x32
+//                 tile, into a per-split partial (no atomics: deterministic).
+//   k_mlp_bgrad   bias, output-layer weight and output-bias partials.
+//   k_mlp_adam    sums the split partials in fixed order and applies Adam.
+//   k_mlp_pack    FP32 master -> BF16 [out][in] and transposed copies.
+//   k_topk        per node, the `keep` best of K scores (radix select on the
+//                 orderable score bits, ties to the lower candidate index),
+//                 written in increasing candidate order.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <math.h>
+
+#pragma clang fp contract(off)
+
+namespace pd {
+namespace mlp {
+
+constexpr int IN = 22;     // logical inputs
+constexpr int INP = 32;    // padded input row (bf16 elements)
+constexpr int HID = 256;
+constexpr int OFF_W1 = 0;
+constexpr int OFF_B1 = OFF_W1 + HID * IN;
+constexpr int OFF_W2 = OFF_B1 + HID;
+constexpr int OFF_B2 = OFF_W2 + HID * HID;
+constexpr int OFF_W3 = OFF_B2 + HID;
+constexpr int OFF_B3 = OFF_W3 + HID * HID;
+constexpr int OFF_W4 = OFF_B3 + HID;
+constexpr int OFF_B4 = OFF_W4 + HID;
+constexpr int NPARAM = OFF_B4 + 1;
+constexpr int ROWS = 64;        // rows per block
+constexpr int THREADS = 256;    // 4 wave64s
+constexpr int LDA = HID + 8;    // LDS row stride in bf16 elements (16 B pad)
+constexpr int SPLIT = 512;      // batch rows per weight-gradient partial
+constexpr int HIST_IN = 16;     // class-histogram inputs
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// round to nearest even (finite inputs)
+__host__ __device__ inline uint16_t f2bf(float f)
+{
+  uint32_t u = __builtin_bit_cast(uint32_t, f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__host__ __device__ inline float bf2f(uint16_t h) { return __builtin_bit_cast(float, (uint32_t)h << 16); }
+
+__device__ __forceinline__ f32x4 mma(uint4 a, uint4 b, f32x4 c)
+{
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c,
+                                                 0, 0, 0);
+}
+
+constexpr float GELU_C = 0.7978845608028654f;   // sqrt(2 / pi)
+constexpr float GELU_A = 0.044715f;
+
+__device__ __forceinline__ float gelu(float z)
+{
+  const float u = GELU_C * (z + GELU_A * z * z * z);
+  return 0.5f * z * (1.0f + tanhf(u));
+}
+__device__ __forceinline__ float gelu_grad(float z)
+{
+  const float z2 = z * z;
+  const float t = tanhf(GELU_C * (z + GELU_A * z2 * z));
+  return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * GELU_C * (1.0f + 3.0f * GELU_A * z2);
+}
+
+// device view of the packed weights
+struct Weights {
+  const uint16_t *W1;    // [256][32] bf16, columns 22..31 zero
+  const uint16_t *W2;    // [256][256]
+  const uint16_t *W3;    // [256][256]
+  const uint16_t *W2T;   // [256][256] transposed
+  const uint16_t *W3T;
+  const uint16_t *w4;    // [256]
+  const float *theta;    // FP32 master (biases are read from it)
+};
+
+// Input features of one (node, candidate) row; the same normalisation for the
+// records the plain trainer emits and for the rows the proposal scores.
+//   hist[c] = n_c / n (c < 16), depth / 16, u, v scaled by the bound of each
+//   component, threshold mapped from [lo, hi] to [-1, 1].
+struct RowNorm {
+  float inv_p[4];
+  float thr_lo, thr_scale;   // thr_n = (thr - lo) * scale - 1, scale = 2 / (hi - lo)
+};
+
+__device__ __forceinline__ void cand_features(const RowNorm &nm, const float *p, float thr, float *out6,
+                                              float depth)
+{
+  out6[0] = depth;
+#pragma unroll
+  for (int j = 0; j < 4; j++) out6[1 + j] = p[j] * nm.inv_p[j];
+  out6[5] = (thr - nm.thr_lo) * nm.thr_scale - 1.0f;
+}
+
+// rows from memory: X bf16 [n][32]
+struct RowsMem {
+  const uint16_t *X;
+  uint32_t n;
+  __device__ __forceinline__ uint4 fill(uint32_t row, uint32_t q) const
+  {
+    if (row >= n) return make_uint4(0u, 0u, 0u, 0u);
+    return *(const uint4 *)(X + (size_t)row * INP + q * 8);
+  }
+};
+
+__device__ __forceinline__ uint4 pack8(const float *v)
+{
+  uint4 r;
+  r.x = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
+  r.y = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
+  r.z = (uint32_t)f2bf(v[4]) | ((uint32_t)f2bf(v[5]) << 16);
+  r.w = (uint32_t)f2bf(v[6]) | ((uint32_t)f2bf(v[7]) << 16);
+  return r;
+}
+
+// Block GEMM of the tile: acc[rt][ct] = act[16 rt + i][:K] . W[64 wave + 16 ct + j][:K]
+// MFMA fragments (16x16x32): lane l holds A[l % 16][8 (l / 16) .. +7],
+// B[8 (l / 16) .. +7][l % 16]; the result C[4 (l / 16) + jj][l % 16] in acc[jj].
+template <int K>
+__device__ __forceinline__ void tile_gemm(const uint16_t *act, const uint16_t *__restrict__ W, uint32_t lane,
+                                          uint32_t wave, f32x4 (&acc)[4][4])
+{
+  const uint32_t r16 = lane & 15u, kg = (lane >> 4) * 8u;
+#pragma unroll
+  for (int rt = 0; rt < 4; rt++)
+#pragma unroll
+    for (int ct = 0; ct < 4; ct++) acc[rt][ct] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll 2
+  for (int kk = 0; kk < K; kk += 32) {
+    uint4 a[4], b[4];
+#pragma unroll
+    for (int ct = 0; ct < 4; ct++)
+      b[ct] = *(const uint4 *)(W + (size_t)(64u * wave + 16u * ct + r16) * K + kk + kg);
+#pragma unroll
+    for (int rt = 0; rt < 4; rt++) a[rt] = *(const uint4 *)(act + (16u * rt + r16) * LDA + kk + kg);
+#pragma unroll
+    for (int rt = 0; rt < 4; rt++)
+#pragma unroll
+      for (int ct = 0; ct < 4; ct++) acc[rt][ct] = mma(a[rt], b[ct], acc[rt][ct]);
+  }
+}
+
+// Forward.  Inference: y[row] for rows < n (grid = ceil(n / 64)).  Training
+// (TRAIN): also Z[l][row][256] (FP32 pre-activations, l = 0..2) and
+// HT[l][256][ldb] (BF16 transposed activations: l = 0 is the input, 32 rows;
+// l = 1..3 the hidden layers); the grid covers ldb rows.
+template <class Src, bool TRAIN>
+__global__ __launch_bounds__(THREADS) void k_mlp_fwd(Src src, uint32_t n, Weights w, float *__restrict__ y,
+                                                     float *__restrict__ Z, uint16_t *__restrict__ HT, uint32_t ldb)
+{
+  __shared__ __align__(16) uint16_t act[ROWS * LDA];
+  const uint32_t tid = threadIdx.x, lane = tid & 63u;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const uint32_t row0 = blockIdx.x * ROWS;
+  {
+    const uint32_t r = tid >> 2, q = tid & 3u;
+    const uint4 v = src.fill(row0 + r, q);
+    *(uint4 *)(act + r * LDA + q * 8) = v;
+    if (TRAIN) {
+      // input transposed: HT[0][k][row], k = 8 q .. 8 q + 7
+      const uint16_t *e = (const uint16_t *)&v;
+#pragma unroll
+      for (int j = 0; j < 8; j++) HT[(size_t)(q * 8 + j) * ldb + row0 + r] = e[j];
+    }
+  }
+  __syncthreads();
+  f32x4 acc[4][4];
+#pragma unroll 1
+  for (int layer = 0; layer < 3; layer++) {
+    const float *bias = w.theta + (layer == 0 ? OFF_B1 : (layer == 1 ? OFF_B2 : OFF_B3));
+    if (layer == 0) tile_gemm<INP>(act, w.W1, lane, wave, acc);
+    else tile_gemm<HID>(act, layer == 1 ? w.W2 : w.W3, lane, wave, acc);
+    __syncthreads();   // every wave has read the layer input
+#pragma unroll
+    for (int rt = 0; rt < 4; rt++)
+#pragma unroll
+      for (int ct = 0; ct < 4; ct++) {
+        const uint32_t col = 64u * wave + 16u * ct + (lane & 15u);
+        const uint32_t rb = 16u * rt + 4u * (lane >> 4);
+        const float b = bias[col];
+        uint16_t hb[4];
+#pragma unroll
+        for (int jj = 0; jj < 4; jj++) {
+          const float z = acc[rt][ct][jj] + b;
+          hb[jj] = f2bf(gelu(z));
+          act[(rb + jj) * LDA + col] = hb[jj];
+          if (TRAIN) Z[((size_t)layer * ldb + row0 + rb + jj) * HID + col] = z;
+        }
+        if (TRAIN) {
+          uint2 pk;
+          pk.x = (uint32_t)hb[0] | ((uint32_t)hb[1] << 16);
+          pk.y = (uint32_t)hb[2] | ((uint32_t)hb[3] << 16);
+          *(uint2 *)(HT + ((size_t)(layer + 1) * HID + col) * ldb + row0 + rb) = pk;
+        }
+      }
+    __syncthreads();
+  }
+  // output layer: y = h3 . w4 + b4 (FP32, fixed summation order)
+  const uint32_t r = tid >> 2, part = tid & 3u;
+  float s = 0.0f;
+#pragma unroll 8
+  for (int k = 0; k < 64; k++) s += bf2f(act[r * LDA + part * 64 + k]) * bf2f(w.w4[part * 64 + k]);
+  s += __shfl_xor(s, 1);
+  s += __shfl_xor(s, 2);
+  if (part == 0 && row0 + r < (TRAIN ? ldb : n)) y[row0 + r] = s + w.theta[OFF_B4];
+}
+
+// Backward of the mean squared error L = 1/n sum_r (y_r - t_r)^2 over the
+// first n rows of the batch (rows n..ldb-1 are padding, dL/dy = 0 there).
+// DZT[l][256][ldb] receives dL/dZ_{l+1} transposed (BF16), dy[ldb] FP32,
+// lossp[block] the block's sum of squared errors.
+__global__ __launch_bounds__(THREADS) void k_mlp_bwd(const float *__restrict__ y, const float *__restrict__ t,
+                                                     uint32_t n, float inv_n, const float *__restrict__ Z,
+                                                     Weights w, uint16_t *__restrict__ DZT, float *__restrict__ dy,
+                                                     float *__restrict__ lossp, uint32_t ldb)
+{
+  __shared__ __align__(16) uint16_t act[ROWS * LDA];
+  __shared__ float dys[ROWS];
+  const uint32_t tid = threadIdx.x, lane = tid & 63u;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const uint32_t row0 = blockIdx.x * ROWS;
+  if (tid < ROWS) {
+    const uint32_t row = row0 + tid;
+    const bool live = row < n;
+    const float e = live ? y[row] - t[row] : 0.0f;
+    const float d = live ? 2.0f * e * inv_n : 0.0f;
+    dys[tid] = d;
+    dy[row] = d;
+    float se = e * e;
+    // fixed-order wave reduction (wave 0 holds the 64 rows)
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) se += __shfl_xor(se, o);
+    if (tid == 0) lossp[blockIdx.x] = se;
+  }
+  __syncthreads();
+  // dZ3[r][k] = dy[r] * w4[k] * gelu'(Z3[r][k]); thread = column k
+  {
+    const uint32_t k = tid;
+    const float wk = bf2f(w.w4[k]);
+    uint16_t *out = DZT + ((size_t)2 * HID + k) * ldb + row0;
+#pragma unroll 1
+    for (int r8 = 0; r8 < ROWS; r8 += 8) {
+      float v[8];
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        const float z = Z[((size_t)2 * ldb + row0 + r8 + j) * HID + k];
+        v[j] = dys[r8 + j] * wk * gelu_grad(z);
+      }
+      const uint4 pk = pack8(v);
+      *(uint4 *)(out + r8) = pk;
+      const uint16_t *e = (const uint16_t *)&pk;
+#pragma unroll
+      for (int j = 0; j < 8; j++) act[(r8 + j) * LDA + k] = e[j];
+    }
+  }
+  __syncthreads();
+  f32x4 acc[4][4];
+#pragma unroll 1
+  for (int layer = 2; layer >= 1; layer--) {
+    // dH_{layer} = dZ_{layer+1} . W_{layer+1}: B[n][k] = W[n][k] = WT[k][n]
+    tile_gemm<HID>(act, layer == 2 ? w.W3T : w.W2T, lane, wave, acc);
+    __syncthreads();
+#pragma unroll
+    for (int rt = 0; rt < 4; rt++)
+#pragma unroll
+      for (int ct = 0; ct < 4; ct++) {
+        const uint32_t col = 64u * wave + 16u * ct + (lane & 15u);
+        const uint32_t rb = 16u * rt + 4u * (lane >> 4);
+        uint16_t hb[4];
+#pragma unroll
+        for (int jj = 0; jj < 4; jj++) {
+          const float z = Z[((size_t)(layer - 1) * ldb + row0 + rb + jj) * HID + col];
+          hb[jj] = f2bf(acc[rt][ct][jj] * gelu_grad(z));
+          act[(rb + jj) * LDA + col] = hb[jj];
+        }
+        uint2 pk;
+        pk.x = (uint32_t)hb[0] | ((uint32_t)hb[1] << 16);
+        pk.y = (uint32_t)hb[2] | ((uint32_t)hb[3] << 16);
+        *(uint2 *)(DZT + ((size_t)(layer - 1) * HID + col) * ldb + row0 + rb) = pk;
+      }
+    __syncthreads();
+  }
+}
+
+// dW[n][k] of one layer over rows [s*SPLIT, (s+1)*SPLIT): DZ = dZ^T [256][ldb],
+// H = input activations transposed [K rows][ldb]; out = partial of split s
+// (row stride k_log; columns >= k_log are not stored).
+// grid (4 n-tiles, k-tiles of 32, splits); wave w: rows n0 + 16 w .. +15.
+__global__ __launch_bounds__(THREADS) void k_mlp_wgrad(const uint16_t *__restrict__ DZ,
+                                                       const uint16_t *__restrict__ H, uint32_t ldb, uint32_t k_log,
+                                                       float *__restrict__ part, size_t part_stride)
+{
+  const uint32_t tid = threadIdx.x, lane = tid & 63u;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const uint32_t n0 = blockIdx.x * 64u + 16u * wave, k0 = blockIdx.y * 32u, s = blockIdx.z;
+  const uint32_t r16 = lane & 15u, kg = (lane >> 4) * 8u;
+  const uint32_t rb = s * SPLIT, re = min(ldb, rb + SPLIT);
+  f32x4 acc[2] = {f32x4{0.0f, 0.0f, 0.0f, 0.0f}, f32x4{0.0f, 0.0f, 0.0f, 0.0f}};
+  const uint16_t *a_row = DZ + (size_t)(n0 + r16) * ldb + kg;
+  const uint16_t *b_row0 = H + (size_t)(k0 + r16) * ldb + kg;
+  const uint16_t *b_row1 = H + (size_t)(k0 + 16u + r16) * ldb + kg;
+  for (uint32_t r = rb; r < re; r += 32) {
+    const uint4 a = *(const uint4 *)(a_row + r);
+    const uint4 b0 = *(const uint4 *)(b_row0 + r);
+    const uint4 b1 = *(const uint4 *)(b_row1 + r);
+    acc[0] = mma(a, b0, acc[0]);
+    acc[1] = mma(a, b1, acc[1]);
+  }
+  float *out = part + (size_t)s * part_stride;
+#pragma unroll
+  for (int ct = 0; ct < 2; ct++) {
+    const uint32_t k = k0 + 16u * ct + r16;
+    if (k >= k_log) continue;
+#pragma unroll
+    for (int jj = 0; jj < 4; jj++) out[(size_t)(n0 + 4u * (lane >> 4) + jj) * k_log + k] = acc[ct][jj];
+  }
+}
+
+// Bias / output-layer partials of split s (blockIdx.y): index i < 768: db of
+// hidden layer i / 256; i < 1024: dw4; i == 1024: db4.
+__global__ __launch_bounds__(THREADS) void k_mlp_bgrad(const uint16_t *__restrict__ DZT,
+                                                       const uint16_t *__restrict__ HT3, const float *__restrict__ dy,
+                                                       uint32_t ldb, float *__restrict__ part, size_t part_stride)
+{
+  const uint32_t i = blockIdx.x * THREADS + threadIdx.x, s = blockIdx.y;
+  if (i > 3u * HID + HID) return;
+  const uint32_t rb = s * SPLIT, re = min(ldb, rb + SPLIT);
+  float acc = 0.0f;
+  float *out = part + (size_t)s * part_stride;
+  if (i < 3u * HID) {
+    const uint32_t l = i / HID, nn = i % HID;
+    const uint16_t *row = DZT + ((size_t)l * HID + nn) * ldb;
+    for (uint32_t r = rb; r < re; r++) acc += bf2f(row[r]);
+    out[(l == 0 ? OFF_B1 : (l == 1 ? OFF_B2 : OFF_B3)) + nn] = acc;
+  } else if (i < 4u * HID) {
+    const uint32_t k = i - 3u * HID;
+    const uint16_t *row = HT3 + (size_t)k * ldb;
+    for (uint32_t r = rb; r < re; r++) acc += dy[r] * bf2f(row[r]);
+    out[OFF_W4 + k] = acc;
+  } else {
+    for (uint32_t r = rb; r < re; r++) acc += dy[r];
+    out[OFF_B4] = acc;
+  }
+}
+
+// g = sum of the split partials (fixed order); optional gradient output;
+// Adam (bias-corrected, as Kingma & Ba 2015, Algorithm 1):
+//   m = b1 m + (1 - b1) g;  v = b2 v + (1 - b2) g^2;
+//   theta -= lr * (m * c1) / (sqrt(v * c2) + eps),  c1 = 1/(1-b1^t), c2 = 1/(1-b2^t)
+__global__ void k_mlp_adam(float *__restrict__ theta, float *__restrict__ m, float *__restrict__ v,
+                           const float *__restrict__ part, uint32_t n_split, size_t part_stride,
+                           float *__restrict__ grad_out, int update, float lr, float b1, float b2, float eps, float c1,
+                           float c2)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (uint32_t)NPARAM) return;
+  float g = 0.0f;
+  for (uint32_t s = 0; s < n_split; s++) g += part[(size_t)s * part_stride + i];
+  if (grad_out) grad_out[i] = g;
+  if (!update) return;
+  const float mi = b1 * m[i] + (1.0f - b1) * g;
+  const float vi = b2 * v[i] + (1.0f - b2) * (g * g);
+  m[i] = mi;
+  v[i] = vi;
+  theta[i] = theta[i] - lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+}
+
+// FP32 master -> BF16 working copies
+__global__ void k_mlp_pack(const float *__restrict__ theta, uint16_t *__restrict__ W1, uint16_t *__restrict__ W2,
+                           uint16_t *__restrict__ W2T, uint16_t *__restrict__ W3, uint16_t *__restrict__ W3T,
+                           uint16_t *__restrict__ w4)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < (uint32_t)(HID * HID)) {
+    const uint32_t nn = i / HID, k = i % HID;
+    const uint16_t a = f2bf(theta[OFF_W2 + i]), b = f2bf(theta[OFF_W3 + i]);
+    W2[i] = a;
+    W3[i] = b;
+    W2T[k * HID + nn] = a;
+    W3T[k * HID + nn] = b;
+  }
+  if (i < (uint32_t)(HID * INP)) {
+    const uint32_t nn = i / INP, k = i % INP;
+    W1[i] = k < (uint32_t)IN ? f2bf(theta[OFF_W1 + nn * IN + k]) : (uint16_t)0;
+  }
+  if (i < (uint32_t)HID) w4[i] = f2bf(theta[OFF_W4 + i]);
+}
+
+// batch gather: rows perm[base + i] of X / t into Xb / tb, zero padding to ldb
+__global__ void k_mlp_gather(const uint16_t *__restrict__ X, const float *__restrict__ t,
+                             const uint32_t *__restrict__ perm, uint64_t base, uint32_t count, uint32_t ldb,
+                             uint16_t *__restrict__ Xb, float *__restrict__ tb)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;   // (row, quarter)
+  const uint32_t r = i >> 2, q = i & 3u;
+  if (r >= ldb) return;
+  uint4 v = make_uint4(0u, 0u, 0u, 0u);
+  float tv = 0.0f;
+  if (r < count) {
+    const uint32_t src = perm[base + r];
+    v = *(const uint4 *)(X + (size_t)src * INP + q * 8);
+    tv = t[src];
+  }
+  *(uint4 *)(Xb + (size_t)r * INP + q * 8) = v;
+  if (q == 0) tb[r] = tv;
+}
+
+// sum of the per-block squared errors of one step (fixed order)
+__global__ void k_mlp_loss(const float *__restrict__ lossp, uint32_t nblk, float scale, float *__restrict__ out)
+{
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  float s = 0.0f;
+  for (uint32_t b = 0; b < nblk; b++) s += lossp[b];
+  *out = s * scale;
+}
+
+// ---------------------------------------------------------------------------
+// Selection: scores [n_slots][K] -> sel [n_slots][keep].  The `screen`
+// largest scores of a slot (ties: lower candidate index first) form its
+// screened set; sel lists the first `keep` members of that set in increasing
+// candidate index (screen == keep: the top keep).  Candidates are i.i.d.
+// draws of the sampler, so with screen > keep the kept ones are a random
+// subset of the screened set.  One 256-thread block per slot.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t order_key(float s)
+{
+  if (s != s) return 0u;   // NaN ranks last
+  const uint32_t u = __builtin_bit_cast(uint32_t, s);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+__global__ __launch_bounds__(256) void k_topk(const float *__restrict__ scores, uint32_t K, uint32_t screen,
+                                              uint32_t keep, uint32_t *__restrict__ sel)
+{
+  __shared__ uint32_t hist[256];
+  __shared__ uint32_t sh_prefix, sh_need;
+  __shared__ uint32_t wsum[2][4];
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+  const float *s = scores + (size_t)blockIdx.x * K;
+  uint32_t *out = sel + (size_t)blockIdx.x * keep;
+  uint32_t prefix = 0u, mask = 0u, need = screen;
+  for (int pass = 0; pass < 4; pass++) {
+    const uint32_t shift = 24u - 8u * (uint32_t)pass;
+    hist[tid] = 0u;
+    __syncthreads();
+    for (uint32_t i = tid; i < K; i += 256) {
+      const uint32_t k = order_key(s[i]);
+      if ((k & mask) == prefix) atomicAdd(&hist[(k >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    if (tid == 0) {
+      // largest digit d with count(> d) < need <= count(>= d)
+      uint32_t above = 0u, d = 255u;
+      for (;; d--) {
+        const uint32_t c = hist[d];
+        if (above + c >= need || d == 0u) break;
+        above += c;
+      }
+      sh_prefix = prefix | (d << shift);
+      sh_need = need - above;
+    }
+    __syncthreads();
+    prefix = sh_prefix;
+    need = sh_need;
+    mask |= 255u << shift;
+  }
+  // prefix = the screen-th largest key; the screened set is every key above
+  // it and the first `need` keys equal to it; the first `keep` of the set in
+  // index order are written
+  uint32_t base_sel = 0u, base_eq = 0u;
+  for (uint32_t c0 = 0; c0 < K; c0 += 256) {
+    const uint32_t i = c0 + tid;
+    const uint32_t k = i < K ? order_key(s[i]) : 0u;
+    const bool gt = i < K && k > prefix, eq = i < K && k == prefix;
+    const unsigned long long meq = __ballot(eq);
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
+    if (lane == 0) wsum[0][wave] = (uint32_t)__popcll(meq);
+    __syncthreads();
+    uint32_t eq_before = base_eq;
+    for (uint32_t w = 0; w < wave; w++) eq_before += wsum[0][w];
+    eq_before += (uint32_t)__popcll(meq & lt);
+    const bool take = gt || (eq && eq_before < need);
+    const unsigned long long mt = __ballot(take);
+    if (lane == 0) wsum[1][wave] = (uint32_t)__popcll(mt);
+    __syncthreads();
+    uint32_t pos = base_sel;
+    for (uint32_t w = 0; w < wave; w++) pos += wsum[1][w];
+    pos += (uint32_t)__popcll(mt & lt);
+    if (take && pos < keep) out[pos] = i;
+    for (uint32_t w = 0; w < 4; w++) {
+      base_eq += wsum[0][w];
+      base_sel += wsum[1][w];
+    }
+    __syncthreads();
+  }
+}
+
+// single-wave layout probe of the MFMA fragments: C[16][16] = A[16][32] B[32][16]
+// (A row-major bf16, B column-major bf16, i.e. Bt[16][32])
+__global__ void k_mma_probe(const uint16_t *__restrict__ A, const uint16_t *__restrict__ Bt, float *__restrict__ C)
+{
+  const uint32_t lane = threadIdx.x;
+  if (lane >= 64) return;
+  const uint32_t r16 = lane & 15u, kg = (lane >> 4) * 8u;
+  const uint4 a = *(const uint4 *)(A + r16 * 32 + kg);
+  const uint4 b = *(const uint4 *)(Bt + r16 * 32 + kg);
+  const f32x4 c = mma(a, b, f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+#pragma unroll
+  for (int jj = 0; jj < 4; jj++) C[(4u * (lane >> 4) + jj) * 16 + r16] = c[jj];
+}
+
+}  // namespace mlp
+}  // namespace pd

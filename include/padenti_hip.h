@@ -50,6 +50,8 @@ extern "C" {
 
 #define PD_MODE_PER_NODE 0      /* one histogram + one split launch per node */
 #define PD_MODE_LEVEL_BATCHED 1 /* one launch per tree level (per slice) */
+#define PD_MODE_PROPOSAL 2      /* level-batched; the split-proposal network picks
+                                   the candidates histogrammed exactly */
 
 /* Mirrors TreeTrainerParameters<FeatType, FeatDim> (src/padenti/tree_trainer.hpp:35-51).
  * Bounds are carried as double and converted to FeatType (exact for float
@@ -81,11 +83,29 @@ typedef struct {
   double route_ms;          /* device time of routing */
   double wall_ms;           /* host wall clock of the level */
   uint32_t split_nodes;
+  uint32_t proposal_nodes;  /* PD_MODE_PROPOSAL: nodes whose pool the network ranked */
+  uint32_t fallback_nodes;  /* PD_MODE_PROPOSAL: trainable nodes evaluated exhaustively */
+  uint32_t judged_nodes;    /* proposal nodes also evaluated exhaustively (judge on) */
+  double mlp_ms;            /* device time of the candidate scoring */
+  double select_ms;         /* device time of the top-k selection */
+  double judge_ms;          /* device time of the exhaustive judge (not training work) */
+  double gain_exhaustive;   /* sum over judged nodes of the exhaustive best FP32 gain */
+  double gain_proposal;     /* sum over judged nodes of the chosen split's FP32 gain */
+  double gain_loss_rel;     /* sum over judged nodes of (g_exh - g_prop) / g_exh, g_exh > 0 */
   uint32_t reserved;
 } pd_level_stats;
 
+/* Adam settings of the split-proposal network. */
+typedef struct {
+  float lr;
+  float beta1;
+  float beta2;
+  float eps;
+} pd_adam;
+
 typedef struct pd_trainer pd_trainer;
 typedef struct pd_forest pd_forest;
+typedef struct pd_mlp pd_mlp;
 
 PD_API const char *pd_last_error(void);
 PD_API int pd_version(void);
@@ -184,6 +204,62 @@ PD_API int pd_copy_bench(int device, uint64_t bytes, int reps, float *ms_per_cop
  * 8 * iters conflict-free 32-bit LDS adds into its own table column (the
  * histogram kernel's access pattern).  *adds_per_s = chip-wide rate. */
 PD_API int pd_lds_atomic_bench(int device, uint32_t iters, int reps, float *ms_per_launch, double *adds_per_s);
+
+/* ---- learned split proposal (no counterpart in the reference) ----------
+ * Network: 22 inputs (node class histogram n_c/n [16, zero padded], depth/16,
+ * candidate u, v scaled by the feature bounds, threshold mapped to [-1, 1])
+ * -> 3 x 256 GELU -> predicted FP32 gain.  BF16 weights / activations, FP32
+ * accumulation (MFMA), FP32 master weights and Adam state.  Flat FP32
+ * parameter order: W1[256][22], b1[256], W2[256][256], b2, W3[256][256], b3,
+ * w4[256], b4 (pd_mlp_n_params() values). */
+PD_API int pd_mlp_create(pd_mlp **out, int device, uint64_t seed);
+PD_API int pd_mlp_destroy(pd_mlp *m);
+PD_API int pd_mlp_n_params(void);
+PD_API int pd_mlp_get_params(pd_mlp *m, float *theta);
+/* replaces the parameters and restarts the Adam state */
+PD_API int pd_mlp_set_params(pd_mlp *m, const float *theta);
+/* host rows x[n][22] (rounded to BF16) -> y[n] */
+PD_API int pd_mlp_forward_host(pd_mlp *m, const float *x, uint32_t n, float *y);
+/* device rows (BF16 [n][32], as pd_mlp_encode_rows writes) -> device y[n] */
+PD_API int pd_mlp_forward_dev(pd_mlp *m, const uint16_t *d_x, uint64_t n, float *d_y, float *kernel_ms);
+/* gradient of L = mean_r (y_r - t_r)^2 over one batch of host rows (n <= 2^20) */
+PD_API int pd_mlp_grad_host(pd_mlp *m, const float *x, const float *t, uint32_t n, float *grad, float *loss);
+/* one Adam step on one batch of host rows; *loss = the batch loss before the step */
+PD_API int pd_mlp_step_host(pd_mlp *m, const float *x, const float *t, uint32_t n, const pd_adam *opt,
+                            float *loss);
+/* `epochs` passes over device records (BF16 rows [n][32], FP32 targets [n])
+ * in shuffled batches; epoch_loss[e] = mean batch loss of epoch e */
+PD_API int pd_mlp_train_dev(pd_mlp *m, const uint16_t *d_x, const float *d_t, uint64_t n, uint32_t batch,
+                            uint32_t epochs, const pd_adam *opt, uint64_t seed, float *epoch_loss, float *ms);
+/* host FP32 rows [n][22] -> BF16 rows [n][32] (the device row format) */
+PD_API int pd_mlp_encode_rows(const float *x, uint64_t n, uint16_t *out);
+/* The proposal's selection alone: for each of n_slots rows of K scores, the
+ * `screen` largest (ties: lower index first) are screened, and the first
+ * `keep` of them in increasing index are listed (screen == keep: top keep). */
+PD_API int pd_topk_select(int device, const float *scores, uint32_t n_slots, uint32_t K, uint32_t screen,
+                          uint32_t keep, uint32_t *sel);
+/* MFMA fragment probe: C[16][16] = A[16][32] . B, B given as Bt[16][32] (BF16) */
+PD_API int pd_mma_probe(int device, const uint16_t *A, const uint16_t *Bt, float *C);
+
+/* Record sink of the plain trainer: while set, every trainable node of a
+ * PD_MODE_LEVEL_BATCHED / PD_MODE_PER_NODE training writes per_node records
+ * (network input row, FP32 gain of that (feature, threshold) pair, sampled
+ * with MD5 of (tree, node, j, 7)) to d_x / d_gain, and, if d_rel is not
+ * NULL, the gain divided by the node's best gain over all its pairs (0 when
+ * that is not positive), up to `capacity` records.  d_x = NULL removes the
+ * sink. */
+PD_API int pd_trainer_set_record_sink(pd_trainer *tr, uint16_t *d_x, float *d_gain, float *d_rel, uint64_t capacity,
+                                      uint32_t per_node);
+PD_API int pd_trainer_records_written(pd_trainer *tr, uint64_t *n);
+/* PD_MODE_PROPOSAL settings: the network (kept alive by the caller); the
+ * number of pool candidates histogrammed exactly (`keep`), taken in sampler
+ * order from the `screen` best-scored (screen = keep or 0: the top keep;
+ * screen = pool size: no use of the scores); the smallest node (samples)
+ * that is ranked (smaller nodes evaluate the whole pool); judge != 0 to
+ * evaluate the whole pool beside every ranked node and report the gain lost
+ * (pd_level_stats). */
+PD_API int pd_trainer_set_proposal(pd_trainer *tr, pd_mlp *mlp, uint32_t keep, uint32_t screen, uint32_t min_samples,
+                                   int judge);
 
 /* ---- device memory helpers (for callers without another allocator) ---- */
 PD_API int pd_dev_alloc(int device, uint64_t bytes, void **out);

@@ -1,0 +1,131 @@
+"""CPU restatement of the split-proposal network.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/ (and nothing in the product
+path).  FP64 forward, backward and Adam of
+
+    x[22] -> z1 = W1 x + b1, h1 = gelu(z1)        (W1: [256][22])
+          -> z2 = W2 h1 + b2, h2 = gelu(z2)       (W2: [256][256])
+          -> z3 = W3 h2 + b3, h3 = gelu(z3)
+          -> y  = w4 . h3 + b4
+    gelu(z) = 0.5 z (1 + tanh(sqrt(2/pi) (z + 0.044715 z^3)))
+    L = mean_r (y_r - t_r)^2
+
+with the flat parameter order W1, b1, W2, b2, W3, b3, w4, b4 of the C ABI.
+With bf16=True the values are rounded to BF16 (round to nearest even) where
+the GPU kernels round them: the inputs, W1..W3 and w4, the hidden
+activations h1..h3 and the hidden-layer gradients dL/dz1..dL/dz3; the GPU
+result then differs from this one only by FP32 accumulation and by rounding
+flips of values within an FP32 rounding error of a BF16 tie.
+
+The network has no counterpart in the reference.  Pinned by: central finite
+differences of forward() against backward() (exact mode), an independent
+forward through torch (float64 CPU nn.functional.linear + gelu(approximate
+='tanh')), torch.optim.Adam for adam_step, and known BF16 rounding cases.
+"""
+import numpy as np
+
+IN, HID = 22, 256
+SIZES = [("W1", (HID, IN)), ("b1", (HID,)), ("W2", (HID, HID)), ("b2", (HID,)), ("W3", (HID, HID)),
+         ("b3", (HID,)), ("w4", (HID,)), ("b4", (1,))]
+NPARAM = sum(int(np.prod(s)) for _, s in SIZES)
+C_GELU = np.sqrt(2.0 / np.pi)
+A_GELU = 0.044715
+
+
+def bf16(x):
+    """Round to the nearest BF16 value (ties to even), returned as float64."""
+    f = np.ascontiguousarray(x, dtype=np.float32)
+    u = f.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def unpack(theta):
+    theta = np.asarray(theta, dtype=np.float64)
+    out, o = {}, 0
+    for name, shape in SIZES:
+        n = int(np.prod(shape))
+        out[name] = theta[o:o + n].reshape(shape)
+        o += n
+    return out
+
+
+def pack(p):
+    return np.concatenate([np.asarray(p[name], dtype=np.float64).reshape(-1) for name, _ in SIZES])
+
+
+def gelu(z):
+    return 0.5 * z * (1.0 + np.tanh(C_GELU * (z + A_GELU * z ** 3)))
+
+
+def gelu_grad(z):
+    t = np.tanh(C_GELU * (z + A_GELU * z ** 3))
+    return 0.5 * (1.0 + t) + 0.5 * z * (1.0 - t * t) * C_GELU * (1.0 + 3.0 * A_GELU * z * z)
+
+
+def _weights(theta, rb):
+    p = unpack(theta)
+    if rb:
+        for k in ("W1", "W2", "W3", "w4"):
+            p[k] = bf16(p[k])
+    return p
+
+
+def forward(theta, x, bf16_mode=False):
+    """y[n] and the cache of activations."""
+    rb = bf16 if bf16_mode else (lambda v: v)
+    p = _weights(theta, bf16_mode)
+    x = rb(np.asarray(x, dtype=np.float64).reshape(-1, IN))
+    z1 = x @ p["W1"].T + p["b1"]
+    h1 = rb(gelu(z1))
+    z2 = h1 @ p["W2"].T + p["b2"]
+    h2 = rb(gelu(z2))
+    z3 = h2 @ p["W3"].T + p["b3"]
+    h3 = rb(gelu(z3))
+    y = h3 @ p["w4"] + p["b4"][0]
+    return y, dict(x=x, z1=z1, h1=h1, z2=z2, h2=h2, z3=z3, h3=h3, p=p)
+
+
+def loss_and_grad(theta, x, t, bf16_mode=False):
+    """(L, dL/dtheta flat) for L = mean (y - t)^2."""
+    rb = bf16 if bf16_mode else (lambda v: v)
+    y, c = forward(theta, x, bf16_mode)
+    t = np.asarray(t, dtype=np.float64).reshape(-1)
+    n = len(t)
+    e = y - t
+    loss = float(np.mean(e * e))
+    dy = 2.0 * e / n
+    p = c["p"]
+    g = {}
+    g["w4"] = c["h3"].T @ dy
+    g["b4"] = np.array([dy.sum()])
+    dz3 = rb(dy[:, None] * p["w4"][None, :] * gelu_grad(c["z3"]))
+    g["W3"] = dz3.T @ c["h2"]
+    g["b3"] = dz3.sum(0)
+    dz2 = rb((dz3 @ p["W3"]) * gelu_grad(c["z2"]))
+    g["W2"] = dz2.T @ c["h1"]
+    g["b2"] = dz2.sum(0)
+    dz1 = rb((dz2 @ p["W2"]) * gelu_grad(c["z1"]))
+    g["W1"] = dz1.T @ c["x"]
+    g["b1"] = dz1.sum(0)
+    return loss, pack(g)
+
+
+def adam_step(theta, m, v, g, step, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+    """One Adam update (Kingma & Ba 2015, Algorithm 1); returns (theta, m, v)."""
+    m = beta1 * m + (1.0 - beta1) * g
+    v = beta2 * v + (1.0 - beta2) * g * g
+    mh = m / (1.0 - beta1 ** step)
+    vh = v / (1.0 - beta2 ** step)
+    return theta - lr * mh / (np.sqrt(vh) + eps), m, v
+
+
+def init_params(seed, scale=1.0):
+    """Random parameters in the layout of the C ABI (tests only)."""
+    rng = np.random.default_rng(seed)
+    p = {}
+    for name, shape in SIZES:
+        fan_in = shape[1] if len(shape) == 2 else (HID if name in ("w4", "b4") else HID)
+        a = scale / np.sqrt(fan_in if name != "b1" else IN)
+        p[name] = rng.uniform(-a, a, size=shape)
+    return pack(p)

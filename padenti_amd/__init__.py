@@ -16,7 +16,7 @@ import numpy as np
 
 from . import _lib as L
 from ._lib import (PD_FEAT_DEMO_S16, PD_FEAT_DEPTH_F32, PD_FEAT_DEPTH_U16, PD_FEAT_RGBD_U16X4,
-                   PD_MODE_LEVEL_BATCHED, PD_MODE_PER_NODE, FEATURES, PdError, check, lib, ptr)
+                   PD_MODE_LEVEL_BATCHED, PD_MODE_PER_NODE, PD_MODE_PROPOSAL, FEATURES, PdError, check, lib, ptr)
 
 __all__ = [
     "TrainingSetImage", "TrainingSet", "Tree", "TreeTrainerParameters", "HIPTreeTrainer",
@@ -189,6 +189,36 @@ class HIPTreeTrainer:
 
     def set_chunk(self, samples_per_chunk):
         check(lib().pd_trainer_set_chunk(self._h, int(samples_per_chunk)))
+
+    def set_record_sink(self, buf, per_node):
+        """Plain trainings emit per_node records per trainable node into the
+        RecordBuffer `buf` (appending after its current count); None removes it."""
+        if buf is None:
+            check(lib().pd_trainer_set_record_sink(self._h, None, None, None, 0, 0))
+            self._sink = None
+            return
+        self._sink = (buf, buf.count)
+        check(lib().pd_trainer_set_record_sink(self._h, buf.x_at(buf.count), buf.t_at(buf.count),
+                                               buf.rel_at(buf.count), buf.capacity - buf.count, int(per_node)))
+
+    def records_written(self):
+        import ctypes
+        n = ctypes.c_uint64()
+        check(lib().pd_trainer_records_written(self._h, ctypes.byref(n)))
+        if getattr(self, "_sink", None):
+            buf, base = self._sink
+            buf.count = base + n.value
+        return n.value
+
+    def set_proposal(self, net, keep, min_samples=0, judge=False, screen=None):
+        """PD_MODE_PROPOSAL: `net` scores the pool of params.n_features
+        candidates (one threshold each) of every node with at least
+        min_samples samples; `keep` of them are histogrammed exactly: the
+        first (in sampler order) of the `screen` best-scored (default: the
+        top keep)."""
+        self._net = net
+        check(lib().pd_trainer_set_proposal(self._h, net._h if net is not None else None, int(keep),
+                                            int(screen or keep), int(min_samples), 1 if judge else 0))
 
     def train(self, tree, ts, params, start_depth, end_depth, mode=PD_MODE_LEVEL_BATCHED, diagnostics=False):
         """TreeTrainer::train(tree, ts, params, startDepth=1, endDepth)."""

@@ -18,6 +18,7 @@ PD_FEAT_RGBD_U16X4 = 2
 PD_FEAT_DEMO_S16 = 3
 PD_MODE_PER_NODE = 0
 PD_MODE_LEVEL_BATCHED = 1
+PD_MODE_PROPOSAL = 2
 
 PD_SYN_BLOBS_F32 = 0
 PD_SYN_CAPSULES_U16 = 1
@@ -67,11 +68,24 @@ class LevelStats(C.Structure):
         ("route_ms", C.c_double),
         ("wall_ms", C.c_double),
         ("split_nodes", C.c_uint32),
+        ("proposal_nodes", C.c_uint32),
+        ("fallback_nodes", C.c_uint32),
+        ("judged_nodes", C.c_uint32),
+        ("mlp_ms", C.c_double),
+        ("select_ms", C.c_double),
+        ("judge_ms", C.c_double),
+        ("gain_exhaustive", C.c_double),
+        ("gain_proposal", C.c_double),
+        ("gain_loss_rel", C.c_double),
         ("reserved", C.c_uint32),
     ]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+class Adam(C.Structure):
+    _fields_ = [("lr", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float)]
 
 
 _lib = None
@@ -120,6 +134,22 @@ def lib():
                                                C.POINTER(C.c_float)]),
         "pd_copy_bench": (I32, [I32, U64, I32, C.POINTER(C.c_float)]),
         "pd_lds_atomic_bench": (I32, [I32, U32, I32, C.POINTER(C.c_float), C.POINTER(C.c_double)]),
+        "pd_mlp_create": (I32, [C.POINTER(VP), I32, U64]),
+        "pd_mlp_destroy": (I32, [VP]),
+        "pd_mlp_n_params": (I32, []),
+        "pd_mlp_get_params": (I32, [VP, VP]),
+        "pd_mlp_set_params": (I32, [VP, VP]),
+        "pd_mlp_forward_host": (I32, [VP, VP, U32, VP]),
+        "pd_mlp_forward_dev": (I32, [VP, VP, U64, VP, C.POINTER(C.c_float)]),
+        "pd_mlp_grad_host": (I32, [VP, VP, VP, U32, VP, C.POINTER(C.c_float)]),
+        "pd_mlp_step_host": (I32, [VP, VP, VP, U32, C.POINTER(Adam), C.POINTER(C.c_float)]),
+        "pd_mlp_train_dev": (I32, [VP, VP, VP, U64, U32, U32, C.POINTER(Adam), U64, VP, C.POINTER(C.c_float)]),
+        "pd_mlp_encode_rows": (I32, [VP, U64, VP]),
+        "pd_mma_probe": (I32, [I32, VP, VP, VP]),
+        "pd_topk_select": (I32, [I32, VP, U32, U32, U32, U32, VP]),
+        "pd_trainer_set_record_sink": (I32, [VP, VP, VP, VP, U64, U32]),
+        "pd_trainer_records_written": (I32, [VP, C.POINTER(U64)]),
+        "pd_trainer_set_proposal": (I32, [VP, VP, U32, U32, U32, I32]),
         "pd_dev_alloc": (I32, [I32, U64, C.POINTER(VP)]),
         "pd_dev_free": (I32, [VP]),
         "pd_memcpy_h2d": (I32, [VP, VP, U64]),

@@ -348,4 +348,210 @@ int pd_dev_sync(int device)
   });
 }
 
+// ---- learned split proposal ------------------------------------------------
+
+int pd_mlp_create(pd_mlp **out, int device, uint64_t seed)
+{
+  return guarded([&] {
+    require(out != nullptr, "null output handle");
+    *out = nullptr;
+    auto m = std::make_unique<pd_mlp>();
+    m->impl = std::make_unique<MlpNet>(device, seed);
+    *out = m.release();
+  });
+}
+
+int pd_mlp_destroy(pd_mlp *m)
+{
+  return guarded([&] { delete m; });
+}
+
+int pd_mlp_n_params(void) { return mlp::NPARAM; }
+
+int pd_mlp_get_params(pd_mlp *m, float *theta)
+{
+  return guarded([&] {
+    require(m && theta, "null argument");
+    m->impl->get_params(theta);
+  });
+}
+
+int pd_mlp_set_params(pd_mlp *m, const float *theta)
+{
+  return guarded([&] {
+    require(m && theta, "null argument");
+    for (int i = 0; i < mlp::NPARAM; i++) require(std::isfinite(theta[i]), "non-finite parameter");
+    m->impl->set_params(theta);
+  });
+}
+
+int pd_mlp_forward_host(pd_mlp *m, const float *x, uint32_t n, float *y)
+{
+  return guarded([&] {
+    require(m && x && y && n >= 1, "bad forward arguments");
+    MlpNet &net = *m->impl;
+    set_device(net.device);
+    std::vector<uint16_t> xb((size_t)n * mlp::INP);
+    MlpNet::encode(x, n, xb.data());
+    net.Xs.ensure(xb.size() * 2);
+    net.ys.ensure((size_t)n * 4);
+    HIPCHK(hipMemcpyAsync(net.Xs.p, xb.data(), xb.size() * 2, hipMemcpyHostToDevice, net.stream));
+    net.forward(net.Xs.as<uint16_t>(), n, net.ys.as<float>(), net.stream);
+    HIPCHK(hipMemcpyAsync(y, net.ys.p, (size_t)n * 4, hipMemcpyDeviceToHost, net.stream));
+    HIPCHK(hipStreamSynchronize(net.stream));
+  });
+}
+
+int pd_mlp_forward_dev(pd_mlp *m, const uint16_t *d_x, uint64_t n, float *d_y, float *kernel_ms)
+{
+  return guarded([&] {
+    require(m && d_x && d_y && n >= 1, "bad forward arguments");
+    MlpNet &net = *m->impl;
+    set_device(net.device);
+    Event a, b;
+    HIPCHK(hipEventRecord(a.e, net.stream));
+    net.forward(d_x, n, d_y, net.stream);
+    HIPCHK(hipEventRecord(b.e, net.stream));
+    HIPCHK(hipEventSynchronize(b.e));
+    if (kernel_ms) *kernel_ms = (float)elapsed_ms(a, b);
+  });
+}
+
+int pd_mlp_grad_host(pd_mlp *m, const float *x, const float *t, uint32_t n, float *grad, float *loss)
+{
+  return guarded([&] {
+    require(m && x && t && grad && n >= 1 && n <= (1u << 20), "bad gradient arguments");
+    MlpNet &net = *m->impl;
+    set_device(net.device);
+    net.stage_host(x, t, n);
+    const pd_adam none{0.0f, 0.9f, 0.999f, 1e-8f};
+    net.step(n, false, none, net.gbuf.as<float>(), net.lbuf.as<float>());
+    HIPCHK(hipMemcpyAsync(grad, net.gbuf.p, mlp::NPARAM * 4, hipMemcpyDeviceToHost, net.stream));
+    float l = 0.0f;
+    HIPCHK(hipMemcpyAsync(&l, net.lbuf.p, 4, hipMemcpyDeviceToHost, net.stream));
+    HIPCHK(hipStreamSynchronize(net.stream));
+    if (loss) *loss = l;
+  });
+}
+
+int pd_mlp_step_host(pd_mlp *m, const float *x, const float *t, uint32_t n, const pd_adam *opt, float *loss)
+{
+  return guarded([&] {
+    require(m && x && t && opt && n >= 1 && n <= (1u << 20), "bad step arguments");
+    require(opt->lr >= 0.0f && opt->beta1 >= 0.0f && opt->beta1 < 1.0f && opt->beta2 >= 0.0f &&
+                opt->beta2 < 1.0f && opt->eps > 0.0f,
+            "bad Adam settings");
+    MlpNet &net = *m->impl;
+    set_device(net.device);
+    net.stage_host(x, t, n);
+    net.step(n, true, *opt, nullptr, net.lbuf.as<float>());
+    float l = 0.0f;
+    HIPCHK(hipMemcpyAsync(&l, net.lbuf.p, 4, hipMemcpyDeviceToHost, net.stream));
+    HIPCHK(hipStreamSynchronize(net.stream));
+    if (loss) *loss = l;
+  });
+}
+
+int pd_mlp_train_dev(pd_mlp *m, const uint16_t *d_x, const float *d_t, uint64_t n, uint32_t batch, uint32_t epochs,
+                     const pd_adam *opt, uint64_t seed, float *epoch_loss, float *ms)
+{
+  return guarded([&] {
+    require(m && opt, "null argument");
+    require(opt->lr >= 0.0f && opt->beta1 >= 0.0f && opt->beta1 < 1.0f && opt->beta2 >= 0.0f &&
+                opt->beta2 < 1.0f && opt->eps > 0.0f,
+            "bad Adam settings");
+    m->impl->train(d_x, d_t, n, batch, epochs, *opt, seed, epoch_loss, ms);
+  });
+}
+
+int pd_mlp_encode_rows(const float *x, uint64_t n, uint16_t *out)
+{
+  return guarded([&] {
+    require(x && out, "null argument");
+    MlpNet::encode(x, n, out);
+  });
+}
+
+int pd_mma_probe(int device, const uint16_t *A, const uint16_t *Bt, float *C)
+{
+  return guarded([&] {
+    require(A && Bt && C, "null argument");
+    set_device(device);
+    DevBuf a, b, c;
+    a.ensure(16 * 32 * 2);
+    b.ensure(16 * 32 * 2);
+    c.ensure(16 * 16 * 4);
+    HIPCHK(hipMemcpy(a.p, A, 16 * 32 * 2, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b.p, Bt, 16 * 32 * 2, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(mlp::k_mma_probe, dim3(1), dim3(64), 0, nullptr, a.as<uint16_t>(), b.as<uint16_t>(),
+                       c.as<float>());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpy(C, c.p, 16 * 16 * 4, hipMemcpyDeviceToHost));
+  });
+}
+
+int pd_topk_select(int device, const float *scores, uint32_t n_slots, uint32_t K, uint32_t screen, uint32_t keep,
+                   uint32_t *sel)
+{
+  return guarded([&] {
+    require(scores && sel && n_slots >= 1 && K >= 1 && keep >= 1 && keep <= screen && screen <= K,
+            "bad top-k arguments");
+    require((uint64_t)n_slots * K < (1ull << 32), "too many scores");
+    set_device(device);
+    DevBuf s, o;
+    s.ensure((size_t)n_slots * K * 4);
+    o.ensure((size_t)n_slots * keep * 4);
+    HIPCHK(hipMemcpy(s.p, scores, (size_t)n_slots * K * 4, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(mlp::k_topk, dim3(n_slots), dim3(256), 0, nullptr, s.as<float>(), K, screen, keep,
+                       o.as<uint32_t>());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpy(sel, o.p, (size_t)n_slots * keep * 4, hipMemcpyDeviceToHost));
+  });
+}
+
+int pd_trainer_set_record_sink(pd_trainer *tr, uint16_t *d_x, float *d_gain, float *d_rel, uint64_t capacity,
+                               uint32_t per_node)
+{
+  return guarded([&] {
+    require(tr != nullptr, "null trainer");
+    if (!d_x) {
+      tr->impl->sink_x = nullptr;
+      tr->impl->sink_t = nullptr;
+      tr->impl->sink_rel = nullptr;
+      tr->impl->sink_cap = tr->impl->sink_n = 0;
+      tr->impl->sink_R = 0;
+      return;
+    }
+    require(d_gain != nullptr && capacity >= 1 && per_node >= 1, "bad record sink");
+    tr->impl->sink_x = d_x;
+    tr->impl->sink_t = d_gain;
+    tr->impl->sink_rel = d_rel;
+    tr->impl->sink_cap = capacity;
+    tr->impl->sink_n = 0;
+    tr->impl->sink_R = per_node;
+  });
+}
+
+int pd_trainer_records_written(pd_trainer *tr, uint64_t *n)
+{
+  return guarded([&] {
+    require(tr && n, "null argument");
+    *n = tr->impl->sink_n;
+  });
+}
+
+int pd_trainer_set_proposal(pd_trainer *tr, pd_mlp *m, uint32_t keep, uint32_t screen, uint32_t min_samples,
+                            int judge)
+{
+  return guarded([&] {
+    require(tr != nullptr, "null trainer");
+    require(m == nullptr || (keep >= 1 && (screen == 0 || screen >= keep)), "need keep >= 1 and screen >= keep");
+    tr->impl->prop_mlp = m ? m->impl.get() : nullptr;
+    tr->impl->prop_keep = keep;
+    tr->impl->prop_screen = screen;
+    tr->impl->prop_min = min_samples;
+    tr->impl->prop_judge = judge;
+  });
+}
+
 }  // extern "C"

@@ -1,0 +1,174 @@
+"""Learned split proposal: the network (pd_mlp_* C ABI), device record
+buffers and the helpers that tie them to the trainer.
+
+    net = SplitProposalNet(seed=1)
+    buf = RecordBuffer(capacity)                # device rows + gains
+    trainer.set_record_sink(buf, per_node=350)  # plain training emits records
+    trainer.train(...)                          # (as many runs as wanted)
+    net.train_records(buf, epochs=2)            # GPU forward/backward/Adam (relative gain)
+    trainer.set_proposal(net, keep=2000, min_samples=256, judge=True)
+    trainer.train(..., mode=PD_MODE_PROPOSAL)   # params.n_features = pool K
+
+Record / input row: [n_c / n for 16 classes, depth / 16, u0/B0, u1/B1,
+v0/B2, v1/B3, (thr - lo) * 2 / (hi - lo) - 1]; targets = the FP32 gain of
+that (feature, threshold) pair measured by the plain trainer, and that gain
+relative to the node's best pair (the network is trained on the relative
+gain: only the order of candidates within a node matters to the proposal).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import check, lib, ptr
+
+IN, INP, HID = 22, 32, 256
+OFF_W1 = 0
+OFF_B1 = OFF_W1 + HID * IN
+OFF_W2 = OFF_B1 + HID
+OFF_B2 = OFF_W2 + HID * HID
+OFF_W3 = OFF_B2 + HID
+OFF_B3 = OFF_W3 + HID * HID
+OFF_W4 = OFF_B3 + HID
+OFF_B4 = OFF_W4 + HID
+NPARAM = OFF_B4 + 1
+
+
+def adam(lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+    return L.Adam(lr, beta1, beta2, eps)
+
+
+class SplitProposalNet:
+    """22 -> 256 -> 256 -> 256 -> 1 GELU network on the GPU (BF16 MFMA)."""
+
+    def __init__(self, seed=1, device=0):
+        h = ctypes.c_void_p()
+        check(lib().pd_mlp_create(ctypes.byref(h), device, int(seed)))
+        self._h = h
+        self.device = device
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pd_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def params(self):
+        out = np.zeros(NPARAM, dtype=np.float32)
+        check(lib().pd_mlp_get_params(self._h, ptr(out)))
+        return out
+
+    @params.setter
+    def params(self, theta):
+        theta = np.ascontiguousarray(theta, dtype=np.float32)
+        assert theta.size == NPARAM
+        check(lib().pd_mlp_set_params(self._h, ptr(theta)))
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, IN)
+        y = np.zeros(len(x), dtype=np.float32)
+        check(lib().pd_mlp_forward_host(self._h, ptr(x), len(x), ptr(y)))
+        return y
+
+    def grad(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, IN)
+        t = np.ascontiguousarray(t, dtype=np.float32).reshape(-1)
+        g = np.zeros(NPARAM, dtype=np.float32)
+        loss = ctypes.c_float()
+        check(lib().pd_mlp_grad_host(self._h, ptr(x), ptr(t), len(x), ptr(g), ctypes.byref(loss)))
+        return g, loss.value
+
+    def step(self, x, t, opt=None):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, IN)
+        t = np.ascontiguousarray(t, dtype=np.float32).reshape(-1)
+        loss = ctypes.c_float()
+        check(lib().pd_mlp_step_host(self._h, ptr(x), ptr(t), len(x), ctypes.byref(opt or adam()),
+                                     ctypes.byref(loss)))
+        return loss.value
+
+    def forward_dev(self, d_x, n, d_y):
+        ms = ctypes.c_float()
+        check(lib().pd_mlp_forward_dev(self._h, d_x, int(n), d_y, ctypes.byref(ms)))
+        return ms.value
+
+    def train_records(self, buf, n=None, batch=4096, epochs=1, opt=None, seed=0, first=0, target="rel"):
+        """Adam passes over records [first, first + n) of a RecordBuffer,
+        regressing the relative gain (target="rel", the ranking target within
+        a node) or the measured gain (target="gain"); returns (per-epoch mean
+        loss, device ms)."""
+        n = buf.count - first if n is None else n
+        loss = np.zeros(epochs, dtype=np.float32)
+        ms = ctypes.c_float()
+        tp = buf.rel_at(first) if target == "rel" else buf.t_at(first)
+        check(lib().pd_mlp_train_dev(self._h, buf.x_at(first), tp, int(n), batch, epochs,
+                                     ctypes.byref(opt or adam()), int(seed), ptr(loss), ctypes.byref(ms)))
+        return loss, ms.value
+
+
+def encode_rows(x):
+    """FP32 rows [n][22] -> BF16 device row format [n][32] (uint16)."""
+    x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, IN)
+    out = np.zeros((len(x), INP), dtype=np.uint16)
+    check(lib().pd_mlp_encode_rows(ptr(x), len(x), ptr(out)))
+    return out
+
+
+def bf16_to_f32(u16):
+    return (np.asarray(u16, dtype=np.uint32) << 16).view(np.float32)
+
+
+class RecordBuffer:
+    """Device storage of records: BF16 rows [capacity][32], FP32 gains and
+    FP32 relative gains (gain / the node's best gain)."""
+
+    def __init__(self, capacity, device=0):
+        self.capacity = int(capacity)
+        self.device = device
+        self.count = 0
+        self._x = ctypes.c_void_p()
+        self._t = ctypes.c_void_p()
+        self._r = ctypes.c_void_p()
+        check(lib().pd_dev_alloc(device, self.capacity * INP * 2, ctypes.byref(self._x)))
+        check(lib().pd_dev_alloc(device, self.capacity * 4, ctypes.byref(self._t)))
+        check(lib().pd_dev_alloc(device, self.capacity * 4, ctypes.byref(self._r)))
+
+    def x_at(self, i):
+        return ctypes.c_void_p(self._x.value + int(i) * INP * 2)
+
+    def t_at(self, i):
+        return ctypes.c_void_p(self._t.value + int(i) * 4)
+
+    def rel_at(self, i):
+        return ctypes.c_void_p(self._r.value + int(i) * 4)
+
+    def upload(self, rows_bf16, t, at=0):
+        rows_bf16 = np.ascontiguousarray(rows_bf16, dtype=np.uint16)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        assert at + len(t) <= self.capacity and rows_bf16.shape == (len(t), INP)
+        check(lib().pd_memcpy_h2d(self.x_at(at), ptr(rows_bf16), rows_bf16.nbytes))
+        check(lib().pd_memcpy_h2d(self.t_at(at), ptr(t), t.nbytes))
+        self.count = max(self.count, at + len(t))
+
+    def download(self, first=0, n=None, rel=False):
+        n = self.count - first if n is None else n
+        x = np.zeros((n, INP), dtype=np.uint16)
+        t = np.zeros(n, dtype=np.float32)
+        r = np.zeros(n, dtype=np.float32)
+        if n:
+            check(lib().pd_memcpy_d2h(ptr(x), self.x_at(first), x.nbytes))
+            check(lib().pd_memcpy_d2h(ptr(t), self.t_at(first), t.nbytes))
+            if rel:
+                check(lib().pd_memcpy_d2h(ptr(r), self.rel_at(first), r.nbytes))
+        return (x, t, r) if rel else (x, t)
+
+    def close(self):
+        for p in (self._x, self._t, self._r):
+            if p and p.value:
+                lib().pd_dev_free(p)
+        self._x = self._t = self._r = ctypes.c_void_p()
+
+    def __del__(self):
+        self.close()
