@@ -435,6 +435,7 @@ struct TrainerBase {
   uint32_t nC = 0;
   uint64_t hist_budget = 8ull << 30;
   uint32_t chunk = 0;
+  pd_tuning tune{8u, 16u};   // waves per histogram block, target blocks per CU
   hipStream_t stream = nullptr;
   // record sink of the plain trainer (device rows bf16 [cap][32], gains [cap])
   uint16_t *sink_x = nullptr;
@@ -458,6 +459,7 @@ struct TrainerBase {
                      float *node_gain, double *node_gain64, pd_level_stats *ls) = 0;
   virtual void hist_nodes(const pd_train_params &P, uint32_t tree_id, uint32_t n_nodes, const int32_t *nodes,
                           const int32_t *sample_node, uint32_t *H_out) = 0;
+  virtual double time_level(const pd_train_params &P, uint32_t tree_id, uint32_t n_nodes, int reps) = 0;
 };
 
 struct SlotInfo {
@@ -591,7 +593,7 @@ struct Trainer final : TrainerBase {
   uint32_t pick_chunk(uint64_t level_samples, uint32_t ftiles) const
   {
     if (chunk) return std::max<uint32_t>(64u, std::min<uint32_t>(chunk, 32768u));
-    const uint64_t target_blocks = (uint64_t)cu_count * 16u;
+    const uint64_t target_blocks = (uint64_t)cu_count * tune.blocks_per_cu;
     uint64_t c = (level_samples * ftiles + target_blocks - 1) / target_blocks;
     c = std::max<uint64_t>(c, 1024u);
     c = std::min<uint64_t>(c, 32768u);
@@ -602,12 +604,21 @@ struct Trainer final : TrainerBase {
   void launch_hist_t(uint32_t n_items, uint32_t item_base, const uint4 *act, const Bounds<F> &bnd, uint32_t tree,
                      uint32_t nF, uint32_t nT)
   {
+    if (tune.waves == 4) launch_hist_w<NTP, 4>(n_items, item_base, act, bnd, tree, nF, nT);
+    else if (tune.waves == 16) launch_hist_w<NTP, 16>(n_items, item_base, act, bnd, tree, nF, nT);
+    else launch_hist_w<NTP, 8>(n_items, item_base, act, bnd, tree, nF, nT);
+  }
+
+  template <int NTP, int NW>
+  void launch_hist_w(uint32_t n_items, uint32_t item_base, const uint4 *act, const Bounds<F> &bnd, uint32_t tree,
+                     uint32_t nF, uint32_t nT)
+  {
     const size_t lds = hist_lds_bytes<F, NTP>(nC);
     if (lds > lds_limit) throw ArgError("histogram tile needs " + std::to_string(lds) + " B of LDS");
     if (lds > 65536)
-      HIPCHK(hipFuncSetAttribute((const void *)k_hist<F, NTP>, hipFuncAttributeMaxDynamicSharedMemorySize,
+      HIPCHK(hipFuncSetAttribute((const void *)k_hist<F, NTP, NW>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)lds));
-    hipLaunchKernelGGL((k_hist<F, NTP>), dim3(n_items), dim3(HIST_THREADS), lds, stream, d_pixels.as<I>(),
+    hipLaunchKernelGGL((k_hist<F, NTP, NW>), dim3(n_items), dim3(NW * 64), lds, stream, d_pixels.as<I>(),
                        d_imgs.as<ImgDesc>(), act, d_items.as<HistItem>(), item_base, d_slot_node.as<int32_t>(),
                        d_slot_mode.as<uint8_t>(), d_slot_hidx.as<int32_t>(), d_tot.as<uint32_t>(), bnd, tree, nF,
                        nT, nC, d_H.as<uint32_t>(), d_cand_key.as<unsigned long long>(), d_cand_left.as<uint32_t>(),
@@ -1197,6 +1208,39 @@ struct Trainer final : TrainerBase {
       if (ls) ls[d - 1] = st;
     }
   }
+
+  // Device time (ms, mean of reps) of one level-batched histogram + split
+  // pass over the training set cut into n_nodes equal consecutive shares
+  // (node ids 0..n_nodes-1), with the current tuning: the autotuner's probe.
+  double time_level(const pd_train_params &P, uint32_t tree_id, uint32_t n_nodes, int reps) override
+  {
+    if (!S) throw StateError("pd_trainer_time_level before pd_trainer_set_data");
+    check_params(P, FD, sizeof(T) == 2);
+    require(n_nodes >= 1 && n_nodes <= S && reps >= 1, "bad level shape");
+    set_device(device);
+    std::vector<SlotInfo> slots(n_nodes);
+    std::vector<uint32_t> classTot((size_t)n_nodes * nC, 0u);
+    for (uint32_t q = 0; q < n_nodes; q++) {
+      const uint64_t b = S * q / n_nodes, e = S * (q + 1) / n_nodes;
+      slots[q] = SlotInfo{(int32_t)q, (uint32_t)b, (uint32_t)(e - b), e > b};
+      for (uint64_t i = b; i < e; i++) classTot[(size_t)q * nC + rec_class_host(h_recs[i])]++;
+    }
+    d_actA.ensure(S * sizeof(uint4));
+    HIPCHK(hipMemcpyAsync(d_actA.p, d_recs.p, S * sizeof(uint4), hipMemcpyDeviceToDevice, stream));
+    const Bounds<F> bnd = make_bounds<F>(P);
+    std::vector<uint32_t> idx(n_nodes), res_all((size_t)n_nodes * (nC + 2));
+    for (uint32_t q = 0; q < n_nodes; q++) idx[q] = q;
+    const uint32_t ch = pick_chunk(S, (P.n_features + 63) / 64);
+    pd_level_stats st{};
+    evaluate(slots, idx, d_actA.as<uint4>(), P, P.n_features, nullptr, nullptr, bnd, tree_id, PD_MODE_LEVEL_BATCHED,
+             ch, classTot, 1, false, res_all, st);   // warm-up (module load, buffers)
+    st = pd_level_stats{};
+    for (int r = 0; r < reps; r++)
+      evaluate(slots, idx, d_actA.as<uint4>(), P, P.n_features, nullptr, nullptr, bnd, tree_id,
+               PD_MODE_LEVEL_BATCHED, ch, classTot, 1, false, res_all, st);
+    return (st.hist_ms + st.split_ms) / reps;
+  }
+  static uint32_t rec_class_host(const uint4 &r) { return r.x >> 24; }
 
   void hist_nodes(const pd_train_params &P, uint32_t tree_id, uint32_t n_nodes, const int32_t *nodes,
                   const int32_t *sample_node, uint32_t *H_out) override

@@ -145,8 +145,8 @@ __device__ __forceinline__ uint32_t prefix_at(const uint32_t *bins, int RW, uint
   return (bins[(c * RW + (l >> 1)) * 64u + lane] >> ((l & 1u) * 16u)) & 0xffffu;
 }
 
-template <class F, int NTP>
-__global__ __launch_bounds__(HIST_THREADS) void k_hist(
+template <class F, int NTP, int NWAVES = HIST_THREADS / 64>
+__global__ __launch_bounds__(NWAVES * 64) void k_hist(
     const typename F::ImgType *__restrict__ pixels, const ImgDesc *__restrict__ imgs,
     const uint4 *__restrict__ recs, const HistItem *__restrict__ items, uint32_t item_base,
     const int32_t *__restrict__ slot_node, const uint8_t *__restrict__ slot_mode,
@@ -157,7 +157,9 @@ __global__ __launch_bounds__(HIST_THREADS) void k_hist(
   using T = typename F::FeatType;
   constexpr int FD = F::FeatDim;
   constexpr int RW = rank_words<NTP>();
-  constexpr uint32_t NW = HIST_THREADS / 64;
+  constexpr uint32_t NW = NWAVES;
+  constexpr uint32_t NTH = NWAVES * 64;
+  static_assert(NWAVES >= 2, "candidate set-up uses two waves");
   constexpr int NS = sorted_slots<NTP>();
   extern __shared__ __align__(16) uint32_t lds[];
   uint32_t *bins = lds;
@@ -178,7 +180,7 @@ __global__ __launch_bounds__(HIST_THREADS) void k_hist(
   // f-th entry of the slot's selected candidate list
   const uint32_t fs = (cand_map && fv) ? cand_map[(size_t)it.slot * nF + f] : f;
 
-  for (uint32_t i = tid; i < nC * RW * 64u; i += HIST_THREADS) bins[i] = 0u;
+  for (uint32_t i = tid; i < nC * RW * 64u; i += NTH) bins[i] = 0u;
 
   // candidate parameters (wave 0) and thresholds (wave 1), one MD5 chain each
   if (tid < 64) {

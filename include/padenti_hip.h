@@ -95,6 +95,12 @@ typedef struct {
   uint32_t reserved;
 } pd_level_stats;
 
+/* Launch tuning of the histogram kernel (chosen by the autotuner). */
+typedef struct {
+  uint32_t waves;          /* wave64s per histogram block: 4, 8 or 16 */
+  uint32_t blocks_per_cu;  /* target work items per CU when a level's samples are cut into chunks */
+} pd_tuning;
+
 /* Adam settings of the split-proposal network. */
 typedef struct {
   float lr;
@@ -145,6 +151,16 @@ PD_API int pd_trainer_train(pd_trainer *tr, const pd_train_params *params, uint3
                      uint32_t end_depth, int mode, const float *root_prior, int32_t *left_children,
                      void *features, void *thresholds, float *posteriors, uint32_t *node_k, float *node_gain,
                      double *node_gain64, pd_level_stats *level_stats);
+
+/* Histogram launch tuning (default {8, 16}); pd_trainer_time_level gives the
+ * device time (ms, mean of reps) of one level-batched histogram + split pass
+ * over the training set cut into n_nodes equal shares with the current
+ * tuning: the autotuner's probe (padenti_amd/autotune.py).  The reference
+ * fixes its work split at compile time (hist_update.cl, local size 16x16). */
+PD_API int pd_trainer_set_tuning(pd_trainer *tr, const pd_tuning *tuning);
+PD_API int pd_trainer_get_tuning(pd_trainer *tr, pd_tuning *tuning);
+PD_API int pd_trainer_time_level(pd_trainer *tr, const pd_train_params *params, uint32_t tree_id, uint32_t n_nodes,
+                                 int reps, double *ms);
 
 /* (a) alone: histograms H[n][c][t][f] (uint32) of the listed nodes, with
  * sample_node[s] giving the node of every packed sample (-1 = none). */

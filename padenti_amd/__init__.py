@@ -154,7 +154,7 @@ class HIPTreeTrainer:
     """Replaces CLTreeTrainer(featurePath, useCPU): the feature is one of the
     compiled HIP hooks, selected by ``feature_kind``."""
 
-    def __init__(self, feature_kind, n_classes, device=0, plugin=None):
+    def __init__(self, feature_kind, n_classes, device=0, plugin=None, tuned=True):
         import ctypes
         self.kind, self.n_classes, self.device = feature_kind, int(n_classes), device
         self.info = plugin.info if plugin is not None else FEATURES[feature_kind]
@@ -166,6 +166,19 @@ class HIPTreeTrainer:
         self._h = h
         self._ts = None
         self.last_level_stats = []
+        # launch tuning from the autotuner's table (built-in hooks only)
+        self._tuning = None
+        if tuned and plugin is None:
+            from .autotune import load_table
+            self._tuning = load_table()
+
+    def _apply_tuning(self, params):
+        if not self._tuning:
+            return
+        from .autotune import key, set_tuning
+        e = self._tuning.get(key(self.kind, params.n_thresholds, self.n_classes))
+        if e:
+            set_tuning(self, e["waves"], e["blocks_per_cu"])
 
     def close(self):
         if getattr(self, "_h", None):
@@ -232,6 +245,7 @@ class HIPTreeTrainer:
         ng64 = np.zeros(n, dtype=np.float64) if diagnostics else None
         stats = (L.LevelStats * max(end_depth - 1, 1))()
         prior = np.ascontiguousarray(ts.priors, dtype=np.float32)
+        self._apply_tuning(params)
         p = params.c()
         check(lib().pd_trainer_train(self._h, p, tree.id, tree.depth, end_depth, mode, ptr(prior),
                                      ptr(tree.left_children), ptr(tree.features), ptr(tree.thresholds),

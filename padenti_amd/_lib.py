@@ -84,6 +84,10 @@ class LevelStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
+class Tuning(C.Structure):
+    _fields_ = [("waves", C.c_uint32), ("blocks_per_cu", C.c_uint32)]
+
+
 class Adam(C.Structure):
     _fields_ = [("lr", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float)]
 
@@ -115,6 +119,9 @@ def lib():
         "pd_trainer_set_data": (I32, [VP, U32, VP, VP, VP, VP, VP, VP, VP, VP, VP]),
         "pd_trainer_set_hist_budget": (I32, [VP, U64]),
         "pd_trainer_set_chunk": (I32, [VP, U32]),
+        "pd_trainer_set_tuning": (I32, [VP, C.POINTER(Tuning)]),
+        "pd_trainer_get_tuning": (I32, [VP, C.POINTER(Tuning)]),
+        "pd_trainer_time_level": (I32, [VP, C.POINTER(TrainParams), U32, U32, I32, C.POINTER(C.c_double)]),
         "pd_trainer_train": (I32, [VP, C.POINTER(TrainParams), U32, U32, U32, I32, VP, VP, VP, VP, VP, VP,
                                    VP, VP, VP]),
         "pd_trainer_hist_nodes": (I32, [VP, C.POINTER(TrainParams), U32, U32, VP, VP, VP]),

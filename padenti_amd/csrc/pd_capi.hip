@@ -65,6 +65,33 @@ int pd_trainer_set_chunk(pd_trainer *tr, uint32_t samples_per_chunk)
   });
 }
 
+int pd_trainer_set_tuning(pd_trainer *tr, const pd_tuning *tuning)
+{
+  return guarded([&] {
+    require(tr && tuning, "null argument");
+    require(tuning->waves == 4 || tuning->waves == 8 || tuning->waves == 16, "waves must be 4, 8 or 16");
+    require(tuning->blocks_per_cu >= 1 && tuning->blocks_per_cu <= 256, "blocks_per_cu must be in [1, 256]");
+    tr->impl->tune = *tuning;
+  });
+}
+
+int pd_trainer_get_tuning(pd_trainer *tr, pd_tuning *tuning)
+{
+  return guarded([&] {
+    require(tr && tuning, "null argument");
+    *tuning = tr->impl->tune;
+  });
+}
+
+int pd_trainer_time_level(pd_trainer *tr, const pd_train_params *params, uint32_t tree_id, uint32_t n_nodes, int reps,
+                          double *ms)
+{
+  return guarded([&] {
+    require(tr && params && ms, "null argument");
+    *ms = tr->impl->time_level(*params, tree_id, n_nodes, reps);
+  });
+}
+
 int pd_trainer_train(pd_trainer *tr, const pd_train_params *params, uint32_t tree_id, uint32_t tree_depth,
                      uint32_t end_depth, int mode, const float *root_prior, int32_t *left_children, void *features,
                      void *thresholds, float *posteriors, uint32_t *node_k, float *node_gain, double *node_gain64,

@@ -61,3 +61,14 @@ def test_tree_xml_short_features(tmp_path):
     treeio.save(t, str(tmp_path / "s.xml"))
     u = treeio.load(str(tmp_path / "s.xml"), pa.PD_FEAT_DEMO_S16, 2, tree_id=3)
     assert np.array_equal(t.features, u.features) and np.array_equal(t.thresholds, u.thresholds)
+
+
+def test_autotune_keys_and_table(tmp_path):
+    from padenti_amd import autotune
+    assert autotune.thr_bucket(1) == 1 and autotune.thr_bucket(20) == 20 and autotune.thr_bucket(21) == 24
+    assert autotune.thr_bucket(100) == 64
+    assert autotune.key(1, 20, 8) == "1:20:8"
+    assert autotune.load_table(str(tmp_path / "missing.json")) == {}
+    path = tmp_path / "t.json"
+    path.write_text('{"entries": [{"key": "1:20:8", "waves": 4, "blocks_per_cu": 32}]}')
+    assert autotune.load_table(str(path))["1:20:8"]["waves"] == 4

@@ -255,6 +255,30 @@ def test_train_c3_shape_bit_exact(gpu):
     tr.close()
 
 
+@pytest.mark.parametrize("waves,bpc", [(4, 4), (16, 64), (8, 1)])
+def test_tuning_does_not_change_the_tree(gpu, waves, bpc):
+    """Launch tuning (waves per block, work items per CU -> chunking, fused
+    vs multi-chunk nodes) changes timing only: the tree is bit-identical."""
+    from padenti_amd import autotune
+    ts = synth.training_set("c2", n_images=10, samples=600)
+    p = synth.params_for("c2", n_features=300)
+    tr = pa.HIPTreeTrainer(pa.PD_FEAT_DEPTH_U16, 8, tuned=False)
+    ref = pa.Tree(1, 10, pa.PD_FEAT_DEPTH_U16, 8)
+    tr.train(ref, ts, p, 1, 10)
+    autotune.set_tuning(tr, waves, bpc)
+    got = pa.Tree(1, 10, pa.PD_FEAT_DEPTH_U16, 8)
+    tr.train(got, ts, p, 1, 10)
+    assert np.array_equal(got.left_children, ref.left_children)
+    assert got.thresholds.tobytes() == ref.thresholds.tobytes()
+    assert got.posteriors.tobytes() == ref.posteriors.tobytes()
+    ms = ctypes.c_double()
+    L.check(L.lib().pd_trainer_time_level(tr._h, p.c(), 0, 16, 1, ctypes.byref(ms)))
+    assert ms.value > 0
+    with pytest.raises(pa.PdError):
+        autotune.set_tuning(tr, 5, 16)
+    tr.close()
+
+
 def test_lds_atomic_probe(gpu):
     ms, rate = ctypes.c_float(), ctypes.c_double()
     pa.check(pa.lib().pd_lds_atomic_bench(0, 64, 2, ctypes.byref(ms), ctypes.byref(rate)))
