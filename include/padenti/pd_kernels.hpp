@@ -70,12 +70,22 @@ constexpr int rank_words() { return (NTP + 2) / 2; }
 template <int NTP>
 constexpr int sorted_slots() { return NTP <= 1 ? 1 : (NTP <= 8 ? 8 : (NTP <= 16 ? 16 : (NTP <= 32 ? 32 : 64))); }
 
+// LDS words of the region shared by the rank bins and, before the bins are
+// zeroed, the candidate staging (parameters + raw thresholds): the staging
+// is dead once the thresholds are sorted, so the two alias.
+template <class F, int NTP>
+constexpr size_t hist_front_words(uint32_t nC)
+{
+  const size_t bins = (size_t)nC * rank_words<NTP>() * 64;
+  const size_t staging = ((size_t)(F::FeatDim + NTP) * 64 * sizeof(typename F::FeatType) + 3) / 4;
+  return bins > staging ? bins : staging;
+}
+
 template <class F, int NTP>
 constexpr size_t hist_lds_bytes(uint32_t nC)
 {
-  return (size_t)nC * rank_words<NTP>() * 64 * 4 +                         // rank bins
+  return hist_front_words<F, NTP>(nC) * 4 +                                 // rank bins | staging
          (size_t)sorted_slots<NTP>() * 64 * sizeof(typename F::FeatType) + // sorted thresholds
-         (size_t)(F::FeatDim + NTP) * 64 * sizeof(typename F::FeatType) +  // staging
          (size_t)NTP * 64;                                                   // ranks L_t
 }
 
@@ -162,11 +172,11 @@ __global__ __launch_bounds__(NWAVES * 64) void k_hist(
   static_assert(NWAVES >= 2, "candidate set-up uses two waves");
   constexpr int NS = sorted_slots<NTP>();
   extern __shared__ __align__(16) uint32_t lds[];
-  uint32_t *bins = lds;
-  T *sth = (T *)(bins + (size_t)nC * RW * 64);
-  T *sp = sth + NS * 64;
-  T *st = sp + FD * 64;
-  uint8_t *sl = (uint8_t *)(st + NTP * 64);
+  uint32_t *bins = lds;           // [nC][RW][64], zeroed once the staging is dead
+  T *sp = (T *)lds;               // staging: parameters [FD][64] ...
+  T *st = sp + FD * 64;           // ... and raw thresholds [NTP][64]
+  T *sth = (T *)(lds + hist_front_words<F, NTP>(nC));
+  uint8_t *sl = (uint8_t *)(sth + NS * 64);
 
   const HistItem it = items[item_base + blockIdx.x];
   if (it.slot == NULL_SLOT) return;   // whole block: no barrier reached
@@ -179,8 +189,6 @@ __global__ __launch_bounds__(NWAVES * 64) void k_hist(
   // sampler index of the lane's candidate: f itself, or (proposal) the
   // f-th entry of the slot's selected candidate list
   const uint32_t fs = (cand_map && fv) ? cand_map[(size_t)it.slot * nF + f] : f;
-
-  for (uint32_t i = tid; i < nC * RW * 64u; i += NTH) bins[i] = 0u;
 
   // candidate parameters (wave 0) and thresholds (wave 1), one MD5 chain each
   if (tid < 64) {
@@ -221,6 +229,8 @@ __global__ __launch_bounds__(NWAVES * 64) void k_hist(
   T p[FD];
 #pragma unroll
   for (int j = 0; j < FD; j++) p[j] = sp[j * 64 + lane];
+  __syncthreads();   // staging read by every wave: the bins may now take its place
+  for (uint32_t i = tid; i < nC * RW * 64u; i += NTH) bins[i] = 0u;
   __syncthreads();
 
   // Sample loop, two samples in flight per wave.  r(v) = #{t : !(v <= thr_t)}
