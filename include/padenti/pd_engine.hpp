@@ -436,6 +436,14 @@ struct TrainerBase {
   uint64_t hist_budget = 8ull << 30;
   uint32_t chunk = 0;
   pd_tuning tune{8u, 16u};   // waves per histogram block, target blocks per CU
+  std::vector<pd_tuning_rule> tune_rules;   // per level, by samples per node (decreasing)
+  pd_tuning level_tuning(uint64_t active, uint32_t trainable) const
+  {
+    const uint64_t spn = trainable ? active / trainable : active;
+    for (const pd_tuning_rule &r : tune_rules)
+      if (spn >= r.min_samples_per_node) return r.tuning;
+    return tune;
+  }
   hipStream_t stream = nullptr;
   // record sink of the plain trainer (device rows bf16 [cap][32], gains [cap])
   uint16_t *sink_x = nullptr;
@@ -1061,6 +1069,10 @@ struct Trainer final : TrainerBase {
         if (s.trainable) active += s.count;
       st.active_samples = active;
       st.evals = (double)active * nF * nT;
+      uint32_t n_trainable = 0;
+      for (const SlotInfo &s : slots) n_trainable += s.trainable ? 1u : 0u;
+      const pd_tuning keep_tune = tune;
+      tune = level_tuning(active, n_trainable);
       uint32_t ch = pick_chunk(active, (nF + 63) / 64);
       std::vector<uint8_t> split_flag(slots.size(), 0);
       // per-slot result: k = t * nF + f (sampler index f), FP32 gain bits, left counts
@@ -1115,6 +1127,7 @@ struct Trainer final : TrainerBase {
         split_flag[q] = 1;
         st.split_nodes++;
       }
+      tune = keep_tune;
       if (d + 1 >= end_depth) {
         st.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
         if (ls) ls[d - 1] = st;

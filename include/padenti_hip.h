@@ -159,6 +159,14 @@ PD_API int pd_trainer_train(pd_trainer *tr, const pd_train_params *params, uint3
  * fixes its work split at compile time (hist_update.cl, local size 16x16). */
 PD_API int pd_trainer_set_tuning(pd_trainer *tr, const pd_tuning *tuning);
 PD_API int pd_trainer_get_tuning(pd_trainer *tr, pd_tuning *tuning);
+/* Per-level tuning: rules sorted by decreasing min_samples_per_node; a level
+ * uses the first rule whose min_samples_per_node <= its active samples per
+ * trainable node (none matching, or n_rules = 0: the single tuning above). */
+typedef struct {
+  uint32_t min_samples_per_node;
+  pd_tuning tuning;
+} pd_tuning_rule;
+PD_API int pd_trainer_set_tuning_rules(pd_trainer *tr, uint32_t n_rules, const pd_tuning_rule *rules);
 PD_API int pd_trainer_time_level(pd_trainer *tr, const pd_train_params *params, uint32_t tree_id, uint32_t n_nodes,
                                  int reps, double *ms);
 

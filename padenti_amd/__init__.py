@@ -178,7 +178,8 @@ class HIPTreeTrainer:
         from .autotune import key, set_tuning
         e = self._tuning.get(key(self.kind, params.n_thresholds, self.n_classes))
         if e:
-            set_tuning(self, e["waves"], e["blocks_per_cu"])
+            rules = [(r["min_samples_per_node"], r["waves"], r["blocks_per_cu"]) for r in e.get("rules", [])]
+            set_tuning(self, e["waves"], e["blocks_per_cu"], rules)
 
     def close(self):
         if getattr(self, "_h", None):

@@ -88,6 +88,10 @@ class Tuning(C.Structure):
     _fields_ = [("waves", C.c_uint32), ("blocks_per_cu", C.c_uint32)]
 
 
+class TuningRule(C.Structure):
+    _fields_ = [("min_samples_per_node", C.c_uint32), ("tuning", Tuning)]
+
+
 class Adam(C.Structure):
     _fields_ = [("lr", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float)]
 
@@ -121,6 +125,7 @@ def lib():
         "pd_trainer_set_chunk": (I32, [VP, U32]),
         "pd_trainer_set_tuning": (I32, [VP, C.POINTER(Tuning)]),
         "pd_trainer_get_tuning": (I32, [VP, C.POINTER(Tuning)]),
+        "pd_trainer_set_tuning_rules": (I32, [VP, U32, VP]),
         "pd_trainer_time_level": (I32, [VP, C.POINTER(TrainParams), U32, U32, I32, C.POINTER(C.c_double)]),
         "pd_trainer_train": (I32, [VP, C.POINTER(TrainParams), U32, U32, U32, I32, VP, VP, VP, VP, VP, VP,
                                    VP, VP, VP]),

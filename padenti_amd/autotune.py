@@ -13,10 +13,16 @@ For a workload it sweeps the launch tuning of the histogram kernel
                                           per CU; it also decides which nodes
                                           are searched inside the kernel)
 
-over level shapes of the real training set (the samples cut into 1, 16, 256
-and 4096 equal nodes: the root, shallow, middle and deep levels of a tree),
-times each (hist + split device time, pd_trainer_time_level) and keeps the
-tuning with the smallest total.  Entries are keyed by (feature hook,
+over level shapes of the real training set (the samples cut into 1, 16,
+256, 1024, 4096 and 16384 equal nodes, as many as leave 64 samples per node:
+the root, shallow, middle and deep levels of a tree) and times each (hist +
+split device time, pd_trainer_time_level).  The entry keeps the tuning with
+the smallest total and, per shape, the fastest one as a per-level rule: a
+level whose active samples per trainable node are nearest (geometrically) to
+a shape's uses that shape's tuning.  Real frontiers are uneven (a few large
+nodes, many small ones), so the choice is then checked end to end: one real
+tree is trained with the default, the best single tuning and the per-level
+rules, and the fastest of the three is stored.  Entries are keyed by (feature hook,
 threshold bucket, classes); HIPTreeTrainer applies the matching entry before
 every train() call.
 """
@@ -33,7 +39,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "tuning.json")
 WAVES = (4, 8, 16)
 BLOCKS_PER_CU = (4, 8, 16, 32, 64)
-SHAPES = (1, 16, 256, 4096)
+SHAPES = (1, 16, 256, 1024, 4096, 16384)
 THR_BUCKETS = (1, 8, 10, 16, 20, 24, 32, 48, 64)
 
 
@@ -55,8 +61,28 @@ def load_table(path=TABLE):
         return {e["key"]: e for e in json.load(f).get("entries", [])}
 
 
-def set_tuning(trainer, waves, blocks_per_cu):
+def set_tuning(trainer, waves, blocks_per_cu, rules=()):
+    """Base tuning + per-level rules [(min_samples_per_node, waves, blocks_per_cu)], decreasing."""
     L.check(L.lib().pd_trainer_set_tuning(trainer._h, ctypes.byref(L.Tuning(waves, blocks_per_cu))))
+    arr = (L.TuningRule * max(len(rules), 1))()
+    for i, (m, w, b) in enumerate(rules):
+        arr[i] = L.TuningRule(int(m), L.Tuning(int(w), int(b)))
+    L.check(L.lib().pd_trainer_set_tuning_rules(trainer._h, len(rules), arr))
+
+
+def rules_from_sweep(res, shapes, S):
+    """Per-level rules: the fastest tuning of each shape, switching at the
+    geometric mean of neighbouring shapes' samples per node."""
+    spn = [S / n for n in shapes]
+    best = [min(res, key=lambda k: res[k][n]) for n in shapes]
+    rules = []
+    for i, (w, b) in enumerate(best):
+        m = int((spn[i] * spn[i + 1]) ** 0.5) if i + 1 < len(shapes) else 0
+        if rules and rules[-1][1:] == (w, b):
+            rules[-1] = (m, w, b)      # merge with the previous (larger) shape's rule
+        else:
+            rules.append((m, w, b))
+    return rules
 
 
 def sweep(trainer, params, shapes=SHAPES, reps=2, log=None):
@@ -92,14 +118,44 @@ def tune_workload(cfg_name, n_images=None, reps=2, log=None):
     total = {k: sum(v.values()) for k, v in res.items()}
     best = min(total, key=total.get)
     default = total[(8, 16)]
+    rules = rules_from_sweep(res, shapes, S)
+    per_shape = sum(min(v[n] for v in res.values()) for n in shapes)
+
+    # End-to-end check on a real tree (real frontiers are far from equal
+    # shares): the default, the best single tuning and the per-level rules;
+    # the fastest is kept.
+    def tree_s(w, b, r):
+        set_tuning(tr, w, b, r)
+        tree = pa.Tree(0, c["depth"], c["kind"], c["n_classes"])
+        tr.train(tree, ts, p, 1, c["depth"])
+        pa.lib().pd_dev_sync(0)
+        t = time.perf_counter()
+        tr.train(tree, ts, p, 1, c["depth"])
+        pa.lib().pd_dev_sync(0)
+        return time.perf_counter() - t
+
+    t_default = tree_s(8, 16, ())
+    t_single = tree_s(best[0], best[1], ())
+    t_rules = tree_s(best[0], best[1], rules)
+    if log:
+        log(f"[autotune] {cfg_name} tree: default {t_default:.3f} s, best single {t_single:.3f} s, "
+            f"per-level rules {t_rules:.3f} s")
+    choice = min((t_default, "default"), (t_single, "single"), (t_rules, "rules"))[1]
+    if choice == "default":
+        best, rules = (8, 16), []
+    elif choice == "single":
+        rules = []
     tr.close()
     return {
         "key": key(c["kind"], p.n_thresholds, c["n_classes"]),
         "workload": f"{cfg_name}: {len(ts.images)} images {c['w']}x{c['h']}, {S} samples, "
                     f"{p.n_features} x {p.n_thresholds}, {c['n_classes']} classes",
         "waves": best[0], "blocks_per_cu": best[1],
-        "total_ms": total[best], "default_total_ms": default, "gain_vs_default": default / total[best],
+        "rules": [{"min_samples_per_node": m, "waves": w, "blocks_per_cu": b} for m, w, b in rules],
+        "total_ms": total[best], "per_level_total_ms": per_shape, "default_total_ms": default,
+        "gain_vs_default": default / total[best], "per_level_gain_vs_default": default / per_shape,
         "shapes": list(shapes),
+        "tree_s": {"default": t_default, "single": t_single, "rules": t_rules, "chosen": choice},
         "sweep": [{"waves": k[0], "blocks_per_cu": k[1], "ms": [v[n] for n in shapes]} for k, v in res.items()],
         "sweep_s": time.perf_counter() - t0,
     }
@@ -118,7 +174,8 @@ def main():
         e = tune_workload(cfg, a.images, a.reps, log)
         table[e["key"]] = e
         log(f"[autotune] {cfg}: waves {e['waves']} blocks/CU {e['blocks_per_cu']} "
-            f"({e['gain_vs_default']:.3f}x vs the default 8 / 16)")
+            f"({e['gain_vs_default']:.3f}x vs the default 8 / 16); per level {e['rules']} "
+            f"({e['per_level_gain_vs_default']:.3f}x)")
     with open(a.out, "w") as f:
         json.dump({"entries": sorted(table.values(), key=lambda e: e["key"])}, f, indent=1)
     print(json.dumps({k: (v["waves"], v["blocks_per_cu"], round(v["gain_vs_default"], 3)) for k, v in table.items()}))

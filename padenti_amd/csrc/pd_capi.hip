@@ -75,6 +75,23 @@ int pd_trainer_set_tuning(pd_trainer *tr, const pd_tuning *tuning)
   });
 }
 
+int pd_trainer_set_tuning_rules(pd_trainer *tr, uint32_t n_rules, const pd_tuning_rule *rules)
+{
+  return guarded([&] {
+    require(tr != nullptr && (n_rules == 0 || rules != nullptr), "null argument");
+    require(n_rules <= 64, "at most 64 tuning rules");
+    std::vector<pd_tuning_rule> v(rules, rules + n_rules);
+    for (size_t i = 0; i < v.size(); i++) {
+      const pd_tuning &t = v[i].tuning;
+      require(t.waves == 4 || t.waves == 8 || t.waves == 16, "waves must be 4, 8 or 16");
+      require(t.blocks_per_cu >= 1 && t.blocks_per_cu <= 256, "blocks_per_cu must be in [1, 256]");
+      require(i == 0 || v[i].min_samples_per_node < v[i - 1].min_samples_per_node,
+              "rules must have decreasing min_samples_per_node");
+    }
+    tr->impl->tune_rules = v;
+  });
+}
+
 int pd_trainer_get_tuning(pd_trainer *tr, pd_tuning *tuning)
 {
   return guarded([&] {

@@ -72,3 +72,12 @@ def test_autotune_keys_and_table(tmp_path):
     path = tmp_path / "t.json"
     path.write_text('{"entries": [{"key": "1:20:8", "waves": 4, "blocks_per_cu": 32}]}')
     assert autotune.load_table(str(path))["1:20:8"]["waves"] == 4
+
+
+def test_autotune_rules_from_sweep():
+    from padenti_amd import autotune
+    shapes = (1, 16, 256)
+    res = {(8, 16): {1: 1.0, 16: 1.0, 256: 2.0}, (16, 64): {1: 0.5, 16: 0.6, 256: 3.0}}
+    rules = autotune.rules_from_sweep(res, shapes, 4096)
+    # 16 waves for 4096 and 256 samples per node, 8 waves below sqrt(256 * 16) = 64
+    assert rules == [(64, 16, 64), (0, 8, 16)]

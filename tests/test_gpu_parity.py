@@ -255,8 +255,9 @@ def test_train_c3_shape_bit_exact(gpu):
     tr.close()
 
 
-@pytest.mark.parametrize("waves,bpc", [(4, 4), (16, 64), (8, 1)])
-def test_tuning_does_not_change_the_tree(gpu, waves, bpc):
+@pytest.mark.parametrize("waves,bpc,rules", [(4, 4, ()), (16, 64, ()), (8, 1, ()),
+                                             (8, 16, ((2000, 16, 64), (300, 4, 8), (0, 8, 1)))])
+def test_tuning_does_not_change_the_tree(gpu, waves, bpc, rules):
     """Launch tuning (waves per block, work items per CU -> chunking, fused
     vs multi-chunk nodes) changes timing only: the tree is bit-identical."""
     from padenti_amd import autotune
@@ -265,7 +266,7 @@ def test_tuning_does_not_change_the_tree(gpu, waves, bpc):
     tr = pa.HIPTreeTrainer(pa.PD_FEAT_DEPTH_U16, 8, tuned=False)
     ref = pa.Tree(1, 10, pa.PD_FEAT_DEPTH_U16, 8)
     tr.train(ref, ts, p, 1, 10)
-    autotune.set_tuning(tr, waves, bpc)
+    autotune.set_tuning(tr, waves, bpc, rules)
     got = pa.Tree(1, 10, pa.PD_FEAT_DEPTH_U16, 8)
     tr.train(got, ts, p, 1, 10)
     assert np.array_equal(got.left_children, ref.left_children)
