@@ -555,6 +555,8 @@ struct Trainer final : TrainerBase {
     for (uint32_t i = 0; i < ni; i++) {
       require(widths[i] >= 1 && widths[i] < 65536 && heights[i] >= 1 && heights[i] < 65536,
               "image size must be in [1, 65535]");
+      require((uint64_t)widths[i] * heights[i] * F::nChannels * sizeof(I) < (1ull << 32),
+              "an image must be smaller than 4 GiB");
       imgs[i].off = img_off[i];
       imgs[i].w = widths[i];
       imgs[i].h = heights[i];
@@ -1422,6 +1424,7 @@ struct Forest final : ForestBase {
   {
     require(!depths.empty(), "forest has no trees");
     require(image && w >= 1 && h >= 1 && w < 65536 && h < 65536, "bad image");
+    require((uint64_t)w * h * F::nChannels * sizeof(I) < (1ull << 32), "an image must be smaller than 4 GiB");
     set_device(device);
     const size_t npx = (size_t)w * h;
     d_img.ensure(npx * F::nChannels * sizeof(I));
@@ -1468,6 +1471,8 @@ struct Forest final : ForestBase {
     uint32_t blocks = 0;
     for (uint32_t i = 0; i < n_frames; i++) {
       require(widths[i] >= 1 && heights[i] >= 1 && widths[i] < 65536 && heights[i] < 65536, "bad frame size");
+      require((uint64_t)widths[i] * heights[i] * F::nChannels * sizeof(I) < (1ull << 32),
+              "a frame must be smaller than 4 GiB");
       fr[i].img_off = img_off[i];
       fr[i].post_off = d_post ? post_off[i] : 0;
       fr[i].leaf_off = d_leaves ? leaf_off[i] : 0;

@@ -34,7 +34,15 @@ struct ImageView {
   int w, h;
   PD_HD ImgT at(int x, int y, int c) const
   {
+#if defined(__HIP_DEVICE_COMPILE__)
+    // 24-bit row multiply (w, y < 2^16) and a 32-bit byte offset from the
+    // image base (images are under 4 GiB, checked when they are uploaded):
+    // full-rate integer ops and a base + 32-bit offset load.
+    const uint32_t i = (__umul24((uint32_t)y, (uint32_t)w) + (uint32_t)x) * (uint32_t)NCH + (uint32_t)c;
+    return *(const ImgT *)((const char *)data + i * (uint32_t)sizeof(ImgT));
+#else
     return data[((uint32_t)y * (uint32_t)w + (uint32_t)x) * (uint32_t)NCH + (uint32_t)c];
+#endif
   }
   PD_HD bool inside(int x, int y) const { return x >= 0 && x < w && y >= 0 && y < h; }
 };
@@ -91,11 +99,20 @@ PD_HD bool div_fast_ok(const DivPrep &dp) { return dp.d >= 0x1p-16f && dp.d <= 0
 // rounds to a value with the same integer part as the exact sum (0.5 is a
 // multiple of ulp(q) there and a rounding never reaches the next integer),
 // so truncation gives round(q); |q| < 0.5 gives 0.
+// The device form clamps with one med3 (a quotient of the hooks is never
+// NaN: d > 0 and finite parameters) and adds copysign(0.5 - 2^-25, q), which
+// needs no |q| < 0.5 case: trunc(q + copysign(0.49999997f, q)) equals the
+// host rule for every float32 with |q| <= 2^20 (checked exhaustively).
 PD_HD int offset_of(float q)
 {
+#if defined(__HIP_DEVICE_COMPILE__)
+  q = __builtin_amdgcn_fmed3f(q, -1048576.0f, 1048576.0f);
+  return (int)(q + copysignf(0.49999997f, q));
+#else
   q = fminf(fmaxf(q, -1048576.0f), 1048576.0f);
   const int n = (int)(q + copysignf(0.5f, q));
   return fabsf(q) < 0.5f ? 0 : n;
+#endif
 }
 
 // Read of channel `ch` at x + round(px/d), y + round(py/d), or `bg` when the

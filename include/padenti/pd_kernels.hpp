@@ -130,17 +130,25 @@ __device__ __forceinline__ void hist_sample(const typename F::ImgType *__restric
   const ImgDesc d = imgs[rec_img(r)];
   ImageView<typename F::ImgType, F::nChannels> view{pixels + d.off, (int)d.w, (int)d.h};
   const T v = F::eval_prep(view, rec_x(r), rec_y(r), unpack_prep<typename F::Prep>(r.z, r.w), p);
-  uint32_t pos = 0;
+  // binary search carried as the byte offset of slot `pos` in the lane's
+  // column (slot stride 64 * sizeof(T)); the step offsets fold into the LDS
+  // read's immediate
+  constexpr uint32_t SB = 64u * (uint32_t)sizeof(T);
+  const char *col = (const char *)(sth + lane);
+  const char *at = col;
 #pragma unroll
   for (int step = NS / 2; step >= 1; step >>= 1) {
-    const T s = sth[(pos + step - 1) * 64u + lane];
-    pos += (v <= s) ? 0u : (uint32_t)step;
+    const T s = *(const T *)(at + (uint32_t)(step - 1) * SB);
+    at += (v <= s) ? 0u : (uint32_t)step * SB;
   }
   // the steps reach at most NS-1: one more test of slot `pos` (the first
   // slot whose predicate can be false) adds the last one when all NS are true
-  if (NS == NTP) pos += (v <= sth[pos * 64u + lane]) ? 0u : 1u;
+  if (NS == NTP) at += (v <= *(const T *)at) ? 0u : SB;
+  const uint32_t pos = (uint32_t)(at - col) / SB;
   const uint32_t rk = pos < nT ? pos : nT;
-  if (fv) atomicAdd(&bins[(rec_cls(r) * RW + (rk >> 1)) * 64u + lane], 1u << ((rk & 1u) * 16u));
+  // class row of the bins: uniform (the record is wave-uniform)
+  const uint32_t crow = __builtin_amdgcn_readfirstlane(rec_cls(r) * (uint32_t)(RW * 64));
+  if (fv) atomicAdd(&bins[crow + (rk >> 1) * 64u + lane], 1u << ((rk & 1u) * 16u));
 }
 
 // Per-slot mode bits of the histogram kernel.

@@ -61,3 +61,18 @@ def test_out_of_image_reads_background():
     assert orc.feature_at(1, img, 2, 2, p) == 10000.0 - 1000.0
     p = np.array([0.0, 0.0, 0.0, -1e9], dtype=np.float32)  # quotient beyond 2^20 -> outside
     assert orc.feature_at(1, img, 2, 2, p) == 1000.0 - 10000.0
+
+
+def test_device_offset_rounding_equals_host_rule_exhaustively():
+    """The device form of the offset rounding, trunc(q + copysign(0.5 - 2^-25, q))
+    without a |q| < 0.5 case, equals the host rule (|q| < 0.5 -> 0, else
+    trunc(q + copysign(0.5, q))) for every float32 in [0, 2^20]; the negative
+    half follows by the sign symmetry of IEEE addition, copysign and trunc."""
+    c = np.float32(0.49999997)
+    assert c == np.nextafter(np.float32(0.5), np.float32(0))
+    hi = int(np.float32(1048576.0).view(np.uint32))
+    for start in range(0, hi + 1, 1 << 26):
+        q = np.arange(start, min(start + (1 << 26), hi + 1), dtype=np.uint32).view(np.float32)
+        host = np.where(q < np.float32(0.5), np.float32(0), np.trunc(q + np.float32(0.5)))
+        dev = np.trunc(q + c)
+        assert np.array_equal(host, dev), start
