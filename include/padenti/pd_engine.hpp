@@ -1332,12 +1332,15 @@ struct Forest final : ForestBase {
   using T = typename F::FeatType;
   using I = typename F::ImgType;
   using Rec = NodeRec<T, F::FeatDim>;
+  // compact forest: existing nodes only (breadth first, children adjacent)
+  // and one leaf table per tree (original node id, posterior row)
   std::vector<Rec> h_nodes;
-  std::vector<float> h_posts;
-  std::vector<uint64_t> h_tree_off, h_post_off;
+  std::vector<float> h_leaf_post;
+  std::vector<int32_t> h_leaf_id;
+  std::vector<uint32_t> h_tree_off, h_leaf_off;
   std::vector<uint32_t> depths;
   uint32_t max_depth = 1;
-  DevBuf d_nodes, d_posts, d_tree_off, d_post_off;
+  DevBuf d_nodes, d_leaf_post, d_leaf_id, d_tree_off, d_leaf_off;
   DevBuf d_img, d_mask, d_out, d_leaf, d_frames;
   // captured graph for predict_frames (keyed by the launch arguments)
   hipGraphExec_t gexec = nullptr;
@@ -1370,26 +1373,44 @@ struct Forest final : ForestBase {
     for (size_t i = 0; i < N; i++)
       if (left[i] >= 0) require((size_t)left[i] == 2 * i + 1 && 2 * i + 2 < N, "malformed tree: bad leftChild");
     set_device(device);
-    h_tree_off.push_back(h_nodes.size());
-    h_post_off.push_back(h_posts.size());
-    for (size_t i = 0; i < N; i++) {
+    // compact the heap layout (cl_classifier_impl.hpp:153-168 uploads the
+    // whole 2^D - 1 arrays): breadth-first walk of the existing nodes, a
+    // split node's children placed side by side
+    const uint32_t base = (uint32_t)h_nodes.size(), leaf_base = (uint32_t)h_leaf_id.size();
+    require((uint64_t)h_nodes.size() + N < (1ull << 31), "forest too large");
+    std::vector<int64_t> order{0};   // original ids in compact order
+    for (size_t k = 0; k < order.size(); k++) {
+      const int64_t i = order[k];
       Rec r{};
-      r.left = left[i];
-      r.thr = thrs[i];
-      for (int j = 0; j < F::FeatDim; j++) r.p[j] = feats[i * F::FeatDim + j];
+      if (left[i] >= 0) {
+        r.left = (int32_t)order.size();
+        order.push_back(2 * i + 1);
+        order.push_back(2 * i + 2);
+        r.thr = thrs[i];
+        for (int j = 0; j < F::FeatDim; j++) r.p[j] = feats[i * F::FeatDim + j];
+      } else {
+        const uint32_t leaf = (uint32_t)h_leaf_id.size() - leaf_base;
+        r.left = -1 - (int32_t)leaf;
+        h_leaf_id.push_back((int32_t)i);
+        h_leaf_post.insert(h_leaf_post.end(), posts + (size_t)i * nC, posts + (size_t)(i + 1) * nC);
+        h_leaf_post.resize(h_leaf_post.size() + (((nC + 3u) & ~3u) - nC), 0.0f);   // row padding
+      }
       h_nodes.push_back(r);
     }
-    h_posts.insert(h_posts.end(), posts, posts + N * nC);
+    h_tree_off.push_back(base);
+    h_leaf_off.push_back(leaf_base);
     depths.push_back(depth);
     max_depth = std::max(max_depth, depth);
     d_nodes.ensure(h_nodes.size() * sizeof(Rec));
-    d_posts.ensure(h_posts.size() * sizeof(float));
-    d_tree_off.ensure(h_tree_off.size() * sizeof(uint64_t));
-    d_post_off.ensure(h_post_off.size() * sizeof(uint64_t));
+    d_leaf_post.ensure(h_leaf_post.size() * sizeof(float));
+    d_leaf_id.ensure(h_leaf_id.size() * sizeof(int32_t));
+    d_tree_off.ensure(h_tree_off.size() * sizeof(uint32_t));
+    d_leaf_off.ensure(h_leaf_off.size() * sizeof(uint32_t));
     HIPCHK(hipMemcpy(d_nodes.p, h_nodes.data(), h_nodes.size() * sizeof(Rec), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(d_posts.p, h_posts.data(), h_posts.size() * sizeof(float), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(d_tree_off.p, h_tree_off.data(), h_tree_off.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(d_post_off.p, h_post_off.data(), h_post_off.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_leaf_post.p, h_leaf_post.data(), h_leaf_post.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_leaf_id.p, h_leaf_id.data(), h_leaf_id.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_tree_off.p, h_tree_off.data(), h_tree_off.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_leaf_off.p, h_leaf_off.data(), h_leaf_off.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     if (gexec) {
       (void)hipGraphExecDestroy(gexec);
       gexec = nullptr;
@@ -1406,8 +1427,8 @@ struct Forest final : ForestBase {
               uint32_t tree0, uint32_t ntr, uint32_t steps, float *out_post, int32_t *out_leaf)
   {
     hipLaunchKernelGGL((k_predict<F, NCM>), dim3(blocks), dim3(256), 0, stream, pix, frames, n_frames, masks,
-                       d_nodes.as<Rec>(), d_tree_off.as<uint64_t>() + tree0, ntr, steps, d_posts.as<float>(),
-                       d_post_off.as<uint64_t>() + tree0, nC, out_post, out_leaf);
+                       d_nodes.as<Rec>(), d_tree_off.as<uint32_t>() + tree0, ntr, steps, d_leaf_post.as<float>(),
+                       d_leaf_id.as<int32_t>(), d_leaf_off.as<uint32_t>() + tree0, nC, out_post, out_leaf);
     HIPCHK(hipGetLastError());
   }
   void launch_any(uint32_t blocks, const I *pix, const FrameDesc *frames, uint32_t n_frames, const uint8_t *masks,

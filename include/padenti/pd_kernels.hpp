@@ -661,6 +661,10 @@ __global__ void k_route_counts(const unsigned long long *__restrict__ rank, cons
 // (cl_classifier_impl.hpp:533-552); output is planar [c][y][x] per frame.
 // Masked-out pixels get posterior 0 and leaf id -1.
 // ---------------------------------------------------------------------------
+// Compact forest node: the tree's existing nodes in breadth-first order,
+// children adjacent.  left >= 0: compact index of the left child (the right
+// child follows it); left < 0: leaf number -1 - left in the tree's leaf
+// table (original node id + posterior row).
 template <typename T, int FD>
 struct alignas(FD * sizeof(T) + 8 <= 16 ? 16 : 32) NodeRec {
   int32_t left;
@@ -682,9 +686,9 @@ template <class F, int NCM>
 __global__ __launch_bounds__(256) void k_predict(
     const typename F::ImgType *__restrict__ pixels, const FrameDesc *__restrict__ frames, uint32_t n_frames,
     const uint8_t *__restrict__ masks, const NodeRec<typename F::FeatType, F::FeatDim> *__restrict__ nodes,
-    const uint64_t *__restrict__ tree_off, uint32_t n_trees, uint32_t max_steps,
-    const float *__restrict__ posts, const uint64_t *__restrict__ post_off, uint32_t nC,
-    float *__restrict__ out_post, int32_t *__restrict__ out_leaf)
+    const uint32_t *__restrict__ tree_off, uint32_t n_trees, uint32_t max_steps,
+    const float *__restrict__ leaf_post, const int32_t *__restrict__ leaf_id, const uint32_t *__restrict__ leaf_off,
+    uint32_t nC, float *__restrict__ out_post, int32_t *__restrict__ out_leaf)
 {
   using T = typename F::FeatType;
   // frame of this block: last frame with block0 <= blockIdx.x (uniform)
@@ -705,24 +709,39 @@ __global__ __launch_bounds__(256) void k_predict(
 #pragma unroll
   for (int c = 0; c < NCM; c++) acc[c] = 0.0f;
   for (uint32_t t = 0; t < n_trees; t++) {
-    int32_t n = -1;
+    int32_t id = -1;
     if (on) {
       const NodeRec<T, F::FeatDim> *tr = nodes + tree_off[t];
-      n = 0;
-      for (uint32_t s = 0; s < max_steps; s++) {
+      int32_t n = 0, leaf = 0;
+      // max_steps splits at most, then the leaf (every node of a tree's
+      // last level is a leaf)
+      for (uint32_t s = 0; s <= max_steps; s++) {
         const NodeRec<T, F::FeatDim> rec = tr[n];
-        if (rec.left < 0) break;
+        if (rec.left < 0) {
+          leaf = -1 - rec.left;
+          break;
+        }
         const T v = F::eval_prep(view, x, y, prep, rec.p);
         n = rec.left + ((v <= rec.thr) ? 0 : 1);
       }
+      const uint32_t li = leaf_off[t] + (uint32_t)leaf;
+      id = leaf_id[li];
       if (out_post) {
-        const float *pp = posts + post_off[t] + (size_t)n * nC;
+        // leaf rows padded to a multiple of 4 floats: 16-byte loads (the
+        // gather of a scattered leaf row costs per load instruction)
+        const float4 *pp = (const float4 *)(leaf_post + (size_t)li * ((nC + 3u) & ~3u));
 #pragma unroll
-        for (int c = 0; c < NCM; c++)
-          if ((uint32_t)c < nC) acc[c] += pp[c];
+        for (int c4 = 0; c4 < NCM / 4; c4++) {
+          if ((uint32_t)(4 * c4) >= nC) break;
+          const float4 q = pp[c4];
+          acc[4 * c4 + 0] += q.x;
+          acc[4 * c4 + 1] += q.y;
+          acc[4 * c4 + 2] += q.z;
+          acc[4 * c4 + 3] += q.w;
+        }
       }
     }
-    if (out_leaf) out_leaf[fd.leaf_off + (size_t)t * npx + px] = n;
+    if (out_leaf) out_leaf[fd.leaf_off + (size_t)t * npx + px] = id;
   }
   if (out_post) {
     const float nt = (float)n_trees;
