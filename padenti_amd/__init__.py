@@ -77,7 +77,8 @@ class TrainingSetImage:
         lab = self.labels[self.samples]
         lab = lab[lab > 0]
         cnt = np.bincount(lab.astype(np.int64) - 1, minlength=n_classes)[:n_classes].astype(np.float32)
-        self.priors = (cnt / np.float32(len(lab))).astype(np.float32)
+        with np.errstate(invalid="ignore", divide="ignore"):   # no samples: 0/0 as the reference
+            self.priors = (cnt / np.float32(len(lab))).astype(np.float32)
 
 
 class TrainingSet:

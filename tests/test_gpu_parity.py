@@ -422,6 +422,75 @@ def test_predict_frames_graph(gpu):
     clf.close()
 
 
+def _ragged_set(n_classes=4, seed=3):
+    """Images of very different sizes (1x1 up to 97x61), samples on borders
+    and corners (probes leave the image), duplicated samples, and one image
+    without samples."""
+    rng = np.random.default_rng(seed)
+    ts = pa.TrainingSet(n_classes)
+    sizes = [(97, 61), (1, 1), (5, 40), (64, 2), (33, 17), (12, 12)]
+    for i, (w, h) in enumerate(sizes):
+        img = synth.gen_image(0, seed, 9000 + i, max(w, 8), max(h, 8), n_classes)[0][:h, :w].copy()
+        lab = rng.integers(1, n_classes + 1, size=(h, w)).astype(np.uint8)
+        n = 0 if i == 4 else int(rng.integers(5, 60))
+        smp = rng.integers(0, w * h, size=n).astype(np.uint32)
+        if n:
+            corners = np.array([0, w - 1, (h - 1) * w, h * w - 1], np.uint32)
+            smp = np.sort(np.concatenate([smp, corners, smp[:3]]))   # corners + duplicates
+        ts << pa.TrainingSetImage(img, w, h, lab, n_classes, smp)
+    ts.priors = np.full(n_classes, 1.0 / n_classes, np.float32)   # the empty image has no priors of its own
+    return ts
+
+
+@pytest.mark.parametrize("nF,nT", [(1, 1), (65, 2), (100, 3)])
+def test_train_ragged_and_degenerate_shapes(gpu, nF, nT):
+    ts = _ragged_set()
+    p = synth.params_for("c1", n_features=nF, n_thresholds=nT)
+    p.per_leaf_samples_thr = 3
+    tr = pa.HIPTreeTrainer(pa.PD_FEAT_DEPTH_F32, 4)
+    tree = pa.Tree(2, 7, pa.PD_FEAT_DEPTH_F32, 4)
+    diag = tr.train(tree, ts, p, 1, 7, diagnostics=True)
+    out = orc.train_tree(orc.Tset(pa.PD_FEAT_DEPTH_F32, ts), p, ts.priors, 2, 7)
+    _check_tree_equal(tree, out, diag)
+    tr.close()
+
+
+def test_train_pure_root_is_a_leaf(gpu):
+    """Every sample of one class: every split has gain 0 and the oracle and
+    the device agree on what the reference does with it."""
+    ts = synth.training_set("c1", n_images=4, samples=64)
+    for im in ts.images:
+        im.labels[:] = 2
+        im.priors = np.array([0, 1, 0, 0], np.float32)
+    ts.priors = np.array([0, 1, 0, 0], np.float32)
+    p = synth.params_for("c1", n_features=30, n_thresholds=5)
+    tr = pa.HIPTreeTrainer(pa.PD_FEAT_DEPTH_F32, 4)
+    tree = pa.Tree(0, 6, pa.PD_FEAT_DEPTH_F32, 4)
+    diag = tr.train(tree, ts, p, 1, 6, diagnostics=True)
+    out = orc.train_tree(orc.Tset(pa.PD_FEAT_DEPTH_F32, ts), p, ts.priors, 0, 6)
+    _check_tree_equal(tree, out, diag)
+    assert diag["node_gain"][0] == 0.0
+    tr.close()
+
+
+def test_predict_tiny_and_masked_frames(gpu):
+    trees = _c1_forest(n_trees=2, depth=6)
+    clf = pa.HIPClassifier(pa.PD_FEAT_DEPTH_F32, 4)
+    for t in trees:
+        clf << t
+    img = synth.gen_image(0, 0, 7777, 8, 8, 4)[0]
+    for w, h in ((1, 1), (1, 7), (8, 1)):
+        sub = img[:h, :w].copy()
+        post = clf.predict_posterior(sub, w, h)
+        assert post.tobytes() == orc.predict_posterior(0, sub, None, trees, 4).tobytes()
+        for ti, t in enumerate(trees):
+            assert np.array_equal(clf.predict_leaves(ti, sub, w, h), orc.predict_leaves(0, sub, None, t))
+    zero = np.zeros((8, 8), np.uint8)
+    assert not clf.predict_posterior(img, 8, 8, zero).any()
+    assert (clf.predict_leaves(0, img, 8, 8, zero) == -1).all()
+    clf.close()
+
+
 def test_error_paths(gpu):
     tr = pa.HIPTreeTrainer(pa.PD_FEAT_DEPTH_F32, 4)
     tree = pa.Tree(0, 5, pa.PD_FEAT_DEPTH_F32, 4)
