@@ -20,6 +20,10 @@ sample, candidate feature): U = sum S*nF; achieved = U / kernel time against
 the chip's sustained LDS-atomic rate measured in the same run by the
 library's probe kernel (pd_lds_atomic_bench).
 
+extras (after the timed steps, skipped with --no-extras): batched
+prediction Mpix/s and the learned split proposal's speedup / gain loss /
+accuracy, the configuration's other headline figures.
+
 cpu_baseline: the CPU oracle (an op-for-op restatement of the reference
 histogram) timed on this host's cores on a bounded sample of the same
 workload: the root node of the same data set with a subset of the candidate
@@ -99,6 +103,38 @@ def traffic_per_launch(cfg_name):
         return json.load(f).get("traffic_bytes_per_launch")
 
 
+def extras():
+    """The other headline figures of the configuration, measured after the
+    timed steps (each in its own guarded section, so that a failure there is
+    reported and cannot hide the training line):
+      predict:  256 mixed RGB-D frames (76.2 Mpx) through 5 depth-20 trees in
+                one hipGraph launch (padenti_amd/predict_bench.py);
+      proposal: learned split proposal vs exhaustive evaluation of a
+                20000-candidate pool on the same data
+                (padenti_amd/proposal_bench.py)."""
+    res = {}
+    try:
+        from padenti_amd import predict_bench
+        r = predict_bench.run(predict_bench.parser().parse_args(["--reps", "5"]))
+        res["predict"] = {k: r[k] for k in ("value", "unit", "frames", "pixels", "ms_graph_launch",
+                                            "frame_latency_ms", "roofline")}
+        print(f"[bench] predict {r['value']:.0f} Mpix/s", file=sys.stderr, flush=True)
+    except Exception as e:   # reported, never fatal
+        res["predict"] = {"error": repr(e)}
+    try:
+        from padenti_amd import proposal_bench
+        r = proposal_bench.run(proposal_bench.parser().parse_args([]))
+        res["proposal"] = {k: r[k] for k in ("value", "unit", "exhaustive_tree_s", "proposal_tree_s", "pool", "keep",
+                                             "mean_gain_loss", "mean_gain_loss_screened",
+                                             "mean_gain_loss_no_network", "heldout_pixel_accuracy",
+                                             "accuracy_loss_pt", "records", "network_train_s")}
+        print(f"[bench] proposal {r['value']:.2f}x over exhaustive, gain loss {r['mean_gain_loss']:.3f}",
+              file=sys.stderr, flush=True)
+    except Exception as e:
+        res["proposal"] = {"error": repr(e)}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -111,6 +147,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--levels-json", default=None, help="write per-level records to this file")
     ap.add_argument("--per-level", action="store_true", help="print per-level timings and evals/s")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the batched-prediction and split-proposal measurements after the timed steps")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (trees or candidate features "
@@ -221,8 +259,10 @@ def main():
         "lds": {"achieved": lds_a, "peak": lds_p, "unit": "atomic adds/s", "frac": lds_a / lds_p if lds_p else None},
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out), flush=True)
     tr.close()
+    if not args.no_extras and args.config == "c2" and args.images is None:
+        out["extras"] = extras()
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

@@ -63,6 +63,10 @@ def timed_train(tr, ts, p, depth, mode, tree_id=0):
 
 
 def main():
+    print(json.dumps(run(parser().parse_args())), flush=True)
+
+
+def parser():
     ap = argparse.ArgumentParser()
     ap.add_argument("--seeds", type=int, nargs="+", default=[1, 2, 3, 4])
     ap.add_argument("--images", type=int, default=500)
@@ -77,7 +81,11 @@ def main():
     ap.add_argument("--screen", type=int, default=10000,
                     help="also judge the screened selection: keep taken in sampler order from the best `screen`")
     ap.add_argument("--depth", type=int, default=16)
-    a = ap.parse_args()
+    return ap
+
+
+def run(a):
+    """The whole benchmark; returns the result record."""
     c = synth.CONFIGS["c2"]
     kind, nC = c["kind"], c["n_classes"]
     tr = pa.HIPTreeTrainer(kind, nC)
@@ -169,9 +177,9 @@ def main():
                     "tflops": flop / mlp_s / 1e12 if mlp_s else None, "peak_bf16_tflops": 2500.0},
         "levels": levels,
     }
-    print(json.dumps(out), flush=True)
     net.close()
     tr.close()
+    return out
 
 
 if __name__ == "__main__":

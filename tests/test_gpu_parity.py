@@ -491,6 +491,68 @@ def test_predict_tiny_and_masked_frames(gpu):
     clf.close()
 
 
+def test_predict_frames_rgbd_21_classes(gpu):
+    """The batched-prediction workload's path: RGB-D hook, 21 classes (padded
+    leaf rows), mixed frame sizes in one launch replayed from a hipGraph,
+    against the oracle bit for bit."""
+    nC = 21
+    ts = synth.training_set("c4", n_images=12, samples=400, w=96, h=72)
+    p = synth.params_for("c4", n_features=120, n_thresholds=10)
+    tr = pa.HIPTreeTrainer(pa.PD_FEAT_RGBD_U16X4, nC)
+    trees = []
+    for t in range(3):
+        tree = pa.Tree(t, 10, pa.PD_FEAT_RGBD_U16X4, nC)
+        tr.train(tree, ts, p, 1, 10)
+        trees.append(tree)
+    tr.close()
+    assert all((t.left_children >= 0).sum() > 20 for t in trees)
+    clf = pa.HIPClassifier(pa.PD_FEAT_RGBD_U16X4, nC)
+    for t in trees:
+        clf << t
+    sizes = [(120, 90), (33, 21), (64, 48), (1, 1), (200, 30)]
+    frames = [synth.gen_image(synth.CONFIGS["c4"]["scene"], 7, 6000 + i, w, h, nC)[0] for i, (w, h) in enumerate(sizes)]
+    npx = np.array([w * h for w, h in sizes], np.uint64)
+    first = np.concatenate([[0], np.cumsum(npx)[:-1]]).astype(np.uint64)
+    img_off, post_off, leaf_off = first * 4, first * nC, first * 3
+    host = np.concatenate([f.reshape(-1) for f in frames]).astype(np.uint16)
+    lib = L.lib()
+
+    def dalloc(nbytes):
+        q = ctypes.c_void_p()
+        L.check(lib.pd_dev_alloc(0, int(nbytes), ctypes.byref(q)))
+        return q
+
+    total = int(npx.sum())
+    d_pix, d_post, d_leaf = dalloc(host.nbytes), dalloc(total * nC * 4), dalloc(total * 3 * 4)
+    L.check(lib.pd_memcpy_h2d(d_pix, L.ptr(host), host.nbytes))
+    w = np.array([s[0] for s in sizes], np.uint32)
+    h = np.array([s[1] for s in sizes], np.uint32)
+    ms = ctypes.c_float()
+    outs = []
+    for use_graph in (1, 1, 0):
+        L.check(lib.pd_forest_predict_frames_dev(clf._h, len(sizes), d_pix, L.ptr(img_off), L.ptr(w), L.ptr(h),
+                                                 None, None, d_post, L.ptr(post_off), d_leaf, L.ptr(leaf_off),
+                                                 use_graph, ctypes.byref(ms)))
+        post = np.zeros(total * nC, np.float32)
+        leaf = np.zeros(total * 3, np.int32)
+        L.check(lib.pd_memcpy_d2h(L.ptr(post), d_post, post.nbytes))
+        L.check(lib.pd_memcpy_d2h(L.ptr(leaf), d_leaf, leaf.nbytes))
+        outs.append((post, leaf))
+    for post, leaf in outs[1:]:
+        assert post.tobytes() == outs[0][0].tobytes() and np.array_equal(leaf, outs[0][1])
+    post, leaf = outs[0]
+    for i, (fw, fh) in enumerate(sizes):
+        n = int(npx[i])
+        want = orc.predict_posterior(pa.PD_FEAT_RGBD_U16X4, frames[i], None, trees, nC).reshape(-1)
+        assert post[int(post_off[i]):int(post_off[i]) + nC * n].tobytes() == want.tobytes(), i
+        for t in range(3):
+            wl = orc.predict_leaves(pa.PD_FEAT_RGBD_U16X4, frames[i], None, trees[t]).reshape(-1)
+            assert np.array_equal(leaf[int(leaf_off[i]) + t * n:int(leaf_off[i]) + (t + 1) * n], wl)
+    for q in (d_pix, d_post, d_leaf):
+        lib.pd_dev_free(q)
+    clf.close()
+
+
 def test_error_paths(gpu):
     tr = pa.HIPTreeTrainer(pa.PD_FEAT_DEPTH_F32, 4)
     tree = pa.Tree(0, 5, pa.PD_FEAT_DEPTH_F32, 4)
