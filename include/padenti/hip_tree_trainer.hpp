@@ -21,6 +21,57 @@ inline void check(int rc)
 }
 }  // namespace pd_cxx
 
+// The learned split-proposal network (no counterpart in the reference):
+// owns a pd_mlp; records live in device memory the caller allocates.
+class SplitProposalNet
+{
+private:
+  pd_mlp *m_net;
+  SplitProposalNet(const SplitProposalNet &);
+  SplitProposalNet &operator=(const SplitProposalNet &);
+
+public:
+  explicit SplitProposalNet(uint64_t seed = 1, int device = 0) : m_net(NULL)
+  {
+    pd_cxx::check(pd_mlp_create(&m_net, device, seed));
+  }
+  ~SplitProposalNet()
+  {
+    if (m_net) pd_mlp_destroy(m_net);
+  }
+  pd_mlp *handle() const { return m_net; }
+  std::vector<float> getParams() const
+  {
+    std::vector<float> t((size_t)pd_mlp_n_params());
+    pd_cxx::check(pd_mlp_get_params(m_net, t.data()));
+    return t;
+  }
+  void setParams(const std::vector<float> &t)
+  {
+    if (t.size() != (size_t)pd_mlp_n_params()) throw std::runtime_error("wrong parameter count");
+    pd_cxx::check(pd_mlp_set_params(m_net, t.data()));
+  }
+  // Adam passes over n device records (BF16 rows [n][32], FP32 targets [n]);
+  // returns the mean loss of every epoch.
+  std::vector<float> train(const uint16_t *d_rows, const float *d_targets, uint64_t n, uint32_t batch,
+                           uint32_t epochs, float lr = 1e-3f, uint64_t seed = 0)
+  {
+    std::vector<float> loss(epochs);
+    const pd_adam opt = {lr, 0.9f, 0.999f, 1e-8f};
+    float ms = 0.f;
+    pd_cxx::check(pd_mlp_train_dev(m_net, d_rows, d_targets, n, batch, epochs, &opt, seed, loss.data(), &ms));
+    return loss;
+  }
+  // Predicted gains of host rows x[n][22].
+  std::vector<float> predict(const std::vector<float> &x) const
+  {
+    const uint32_t n = (uint32_t)(x.size() / 22);
+    std::vector<float> y(n);
+    if (n) pd_cxx::check(pd_mlp_forward_host(m_net, x.data(), n, y.data()));
+    return y;
+  }
+};
+
 template <typename ImgType, unsigned int nChannels, typename FeatType, unsigned int FeatDim, unsigned int nClasses>
 class HIPTreeTrainer : public TreeTrainer<ImgType, nChannels, FeatType, FeatDim, nClasses>
 {
@@ -29,6 +80,7 @@ private:
   const void *m_uploaded;
   size_t m_uploadedImages;
   std::vector<pd_level_stats> m_stats;
+  int m_mode;
 
   HIPTreeTrainer(const HIPTreeTrainer &);
   HIPTreeTrainer &operator=(const HIPTreeTrainer &);
@@ -66,14 +118,46 @@ private:
   }
 
 public:
-  explicit HIPTreeTrainer(int featureKind, int device = 0) : m_tr(NULL), m_uploaded(NULL), m_uploadedImages(0)
+  explicit HIPTreeTrainer(int featureKind, int device = 0)
+    : m_tr(NULL), m_uploaded(NULL), m_uploadedImages(0), m_mode(PD_MODE_LEVEL_BATCHED)
   {
     pd_cxx::check(pd_trainer_create(&m_tr, device, featureKind, nClasses));
   }
   explicit HIPTreeTrainer(pd_trainer_factory create, int device = 0)
-    : m_tr(NULL), m_uploaded(NULL), m_uploadedImages(0)
+    : m_tr(NULL), m_uploaded(NULL), m_uploadedImages(0), m_mode(PD_MODE_LEVEL_BATCHED)
   {
     pd_cxx::check(create(&m_tr, device, nClasses));
+  }
+
+  // Histogram launch tuning (padenti_amd/tuning.json holds the autotuner's
+  // choice per workload): waves per block 4 / 8 / 16, work items per CU.
+  void setTuning(uint32_t waves, uint32_t blocksPerCU)
+  {
+    const pd_tuning t = {waves, blocksPerCU};
+    pd_cxx::check(pd_trainer_set_tuning(m_tr, &t));
+  }
+  // One trainer call in three: per-node launches, level-batched (default),
+  // or level-batched with the learned split proposal (setProposal first).
+  void setMode(int mode) { m_mode = mode; }
+  // Records of later plain trainings go to device memory (see pd_trainer_set_record_sink).
+  void setRecordSink(uint16_t *d_rows, float *d_gain, float *d_rel, uint64_t capacity, uint32_t perNode)
+  {
+    pd_cxx::check(pd_trainer_set_record_sink(m_tr, d_rows, d_gain, d_rel, capacity, perNode));
+  }
+  uint64_t recordsWritten()
+  {
+    uint64_t n = 0;
+    pd_cxx::check(pd_trainer_records_written(m_tr, &n));
+    return n;
+  }
+  // PD_MODE_PROPOSAL: params.nFeatures is the pool size (nThresholds 1);
+  // `keep` pool candidates of every node with >= minSamples samples are
+  // histogrammed exactly; judge = also evaluate the whole pool and report
+  // the gain lost in levelStats().
+  void setProposal(const SplitProposalNet &net, uint32_t keep, uint32_t minSamples, bool judge = false,
+                   uint32_t screen = 0)
+  {
+    pd_cxx::check(pd_trainer_set_proposal(m_tr, net.handle(), keep, screen, minSamples, judge ? 1 : 0));
   }
   ~HIPTreeTrainer()
   {
@@ -101,7 +185,7 @@ public:
     p.thr_up = (double)params.thrUpBound;
     p.per_leaf_samples_thr = params.perLeafSamplesThr;
     m_stats.assign(endDepth > 1 ? endDepth - 1 : 1, pd_level_stats());
-    pd_cxx::check(pd_trainer_train(m_tr, &p, tree.getID(), tree.getDepth(), endDepth, PD_MODE_LEVEL_BATCHED,
+    pd_cxx::check(pd_trainer_train(m_tr, &p, tree.getID(), tree.getDepth(), endDepth, m_mode,
                                    trainingSet.getPriors(), tree.getLeftChildren(), tree.getFeatures(),
                                    tree.getThresholds(), tree.getPosteriors(), NULL, NULL, NULL, m_stats.data()));
   }

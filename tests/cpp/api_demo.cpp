@@ -87,6 +87,50 @@ int main(int argc, char **argv)
       threw = true;
     }
     if (!threw) return 1;
+
+    // launch tuning changes timing only
+    {
+      TreeT tuned(0, 6);
+      TrainerT t2(PD_FEAT_DEPTH_F32);
+      t2.setTuning(16, 64);
+      t2.train(tuned, ts, params, 1, 6);
+      if (!same_tree(tree, tuned)) {
+        fprintf(stderr, "tuned launch trained a different tree\n");
+        return 1;
+      }
+    }
+    // learned split proposal: records from a plain training, the network
+    // trained on them, and a proposal that keeps every candidate (= plain)
+    {
+      const uint64_t cap = 1u << 16;
+      void *rows = NULL, *gain = NULL, *rel = NULL;
+      pd_cxx::check(pd_dev_alloc(0, cap * 64, &rows));
+      pd_cxx::check(pd_dev_alloc(0, cap * 4, &gain));
+      pd_cxx::check(pd_dev_alloc(0, cap * 4, &rel));
+      TrainerT t3(PD_FEAT_DEPTH_F32);
+      t3.setRecordSink((uint16_t *)rows, (float *)gain, (float *)rel, cap, 100);
+      TreeT plain(0, 6);
+      t3.train(plain, ts, params, 1, 6);
+      const uint64_t n = t3.recordsWritten();
+      t3.setRecordSink(NULL, NULL, NULL, 0, 0);
+      SplitProposalNet net(3);
+      const std::vector<float> loss = net.train((const uint16_t *)rows, (const float *)rel, n, 1024, 2);
+      if (n < 1000 || !(loss[1] == loss[1]) || !same_tree(tree, plain)) {
+        fprintf(stderr, "record sink / network training failed (%llu records)\n", (unsigned long long)n);
+        return 1;
+      }
+      t3.setProposal(net, params.nFeatures, 0, true);
+      t3.setMode(PD_MODE_PROPOSAL);
+      TreeT prop(0, 6);
+      t3.train(prop, ts, params, 1, 6);
+      if (!same_tree(tree, prop)) {
+        fprintf(stderr, "proposal keeping every candidate trained a different tree\n");
+        return 1;
+      }
+      pd_dev_free(rows);
+      pd_dev_free(gain);
+      pd_dev_free(rel);
+    }
     tree.save(out + "/tree.xml", -1, true);
     tree.save(out + "/tree_default.xml");
     TreeT loaded;
