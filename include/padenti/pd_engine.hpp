@@ -312,7 +312,7 @@ struct MlpNet {
   // inference on device rows (bf16 [n][32]) on stream s
   void forward(const uint16_t *X, uint64_t n, float *y, hipStream_t s)
   {
-    const uint64_t blocks = (n + mlp::ROWS - 1) / mlp::ROWS;
+    const uint64_t blocks = (n + mlp::FWD_ROWS - 1) / mlp::FWD_ROWS;
     require(blocks < (1ull << 31), "too many rows");
     if (!blocks) return;
     hipLaunchKernelGGL((mlp::k_mlp_fwd<mlp::RowsMem, false>), dim3((uint32_t)blocks), dim3(mlp::THREADS), 0, s,
@@ -331,7 +331,7 @@ struct MlpNet {
     const mlp::Weights w = weights();
     uint16_t *ht = HT.as<uint16_t>(), *dz = DZT.as<uint16_t>();
     const size_t hl = (size_t)mlp::HID * ldb;   // one layer of HT / DZT
-    hipLaunchKernelGGL((mlp::k_mlp_fwd<mlp::RowsMem, true>), dim3(nblk), dim3(mlp::THREADS), 0, stream,
+    hipLaunchKernelGGL((mlp::k_mlp_fwd<mlp::RowsMem, true>), dim3(lrow / mlp::FWD_ROWS), dim3(mlp::THREADS), 0, stream,
                        mlp::RowsMem{Xb.as<uint16_t>(), n}, n, w, yb.as<float>(), Z.as<float>(), ht, ldb);
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(mlp::k_mlp_bwd, dim3(nblk), dim3(mlp::THREADS), 0, stream, yb.as<float>(), tb.as<float>(), n,
@@ -971,7 +971,8 @@ struct Trainer final : TrainerBase {
       HIPCHK(hipMemcpyAsync(d_pslot_node.p, snode.data(), G * sizeof(int32_t), hipMemcpyHostToDevice, stream));
       const PoolRows<F> src{d_pslot_in.as<float>(), d_pslot_node.as<int32_t>(), K, tree, (uint32_t)rows, bnd, nm};
       HIPCHK(hipEventRecord(ea.e, stream));
-      hipLaunchKernelGGL((mlp::k_mlp_fwd<PoolRows<F>, false>), dim3((uint32_t)((rows + mlp::ROWS - 1) / mlp::ROWS)),
+      hipLaunchKernelGGL((mlp::k_mlp_fwd<PoolRows<F>, false>),
+                         dim3((uint32_t)((rows + mlp::FWD_ROWS - 1) / mlp::FWD_ROWS)),
                          dim3(mlp::THREADS), 0, stream, src, (uint32_t)rows, prop_mlp->weights(),
                          d_scores.as<float>(), (float *)nullptr, (uint16_t *)nullptr, 0u);
       HIPCHK(hipGetLastError());

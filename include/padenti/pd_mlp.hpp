@@ -7,9 +7,10 @@
 // FP32 master weights and Adam state.  Gradients of the hidden layers are
 // carried in BF16 between the matrix products, as the activations are.
 //
-// Kernels (one 64-row tile per 256-thread block, 4 wave64s):
-//   k_mlp_fwd     all layers fused; the tile's activations stay in LDS.  A
-//                 wave owns 64 output columns: 4 x 4 MFMA 16x16 tiles, the
+// Kernels (256-thread blocks, 4 wave64s; 128-row tiles forward, 64 backward):
+//   k_mlp_fwd     all layers fused, 128 rows per block; the tile's activations
+//                 stay in LDS.  A wave owns 64 output columns: 8 x 4 MFMA
+//                 16x16 tiles, the
 //                 A operand (activations) from LDS, the B operand (weights,
 //                 [out][in] row-major) straight from L2.  Training mode also
 //                 stores the FP32 pre-activations Z and the transposed BF16
@@ -47,7 +48,8 @@ constexpr int OFF_B3 = OFF_W3 + HID * HID;
 constexpr int OFF_W4 = OFF_B3 + HID;
 constexpr int OFF_B4 = OFF_W4 + HID;
 constexpr int NPARAM = OFF_B4 + 1;
-constexpr int ROWS = 64;        // rows per block
+constexpr int ROWS = 64;        // rows per block of the backward pass
+constexpr int FWD_ROWS = 128;   // rows per block of the forward pass (weights re-read from L2 once per tile)
 constexpr int THREADS = 256;    // 4 wave64s
 constexpr int LDA = HID + 8;    // LDS row stride in bf16 elements (16 B pad)
 constexpr int SPLIT = 512;      // batch rows per weight-gradient partial
@@ -74,16 +76,19 @@ __device__ __forceinline__ f32x4 mma(uint4 a, uint4 b, f32x4 c)
 constexpr float GELU_C = 0.7978845608028654f;   // sqrt(2 / pi)
 constexpr float GELU_A = 0.044715f;
 
-__device__ __forceinline__ float gelu(float z)
+// tanh-form GELU through 0.5 (1 + tanh u) = 1 / (1 + e^(-2u)): one exp and
+// one reciprocal instead of tanhf (the same function; FP32 rounding only)
+__device__ __forceinline__ float gelu_sig(float z)
 {
   const float u = GELU_C * (z + GELU_A * z * z * z);
-  return 0.5f * z * (1.0f + tanhf(u));
+  return __builtin_amdgcn_rcpf(1.0f + __expf(-2.0f * u));
 }
+__device__ __forceinline__ float gelu(float z) { return z * gelu_sig(z); }
+// d gelu / dz = s + 2 z s (1 - s) u',  s = sigmoid(2u), u' = c (1 + 3 a z^2)
 __device__ __forceinline__ float gelu_grad(float z)
 {
-  const float z2 = z * z;
-  const float t = tanhf(GELU_C * (z + GELU_A * z2 * z));
-  return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * GELU_C * (1.0f + 3.0f * GELU_A * z2);
+  const float s = gelu_sig(z);
+  return s + 2.0f * z * s * (1.0f - s) * (GELU_C * (1.0f + 3.0f * GELU_A * z * z));
 }
 
 // device view of the packed weights
@@ -139,25 +144,25 @@ __device__ __forceinline__ uint4 pack8(const float *v)
 // Block GEMM of the tile: acc[rt][ct] = act[16 rt + i][:K] . W[64 wave + 16 ct + j][:K]
 // MFMA fragments (16x16x32): lane l holds A[l % 16][8 (l / 16) .. +7],
 // B[8 (l / 16) .. +7][l % 16]; the result C[4 (l / 16) + jj][l % 16] in acc[jj].
-template <int K>
+template <int K, int RT = 4>
 __device__ __forceinline__ void tile_gemm(const uint16_t *act, const uint16_t *__restrict__ W, uint32_t lane,
-                                          uint32_t wave, f32x4 (&acc)[4][4])
+                                          uint32_t wave, f32x4 (&acc)[RT][4])
 {
   const uint32_t r16 = lane & 15u, kg = (lane >> 4) * 8u;
 #pragma unroll
-  for (int rt = 0; rt < 4; rt++)
+  for (int rt = 0; rt < RT; rt++)
 #pragma unroll
     for (int ct = 0; ct < 4; ct++) acc[rt][ct] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll 2
   for (int kk = 0; kk < K; kk += 32) {
-    uint4 a[4], b[4];
+    uint4 a[RT], b[4];
 #pragma unroll
     for (int ct = 0; ct < 4; ct++)
       b[ct] = *(const uint4 *)(W + (size_t)(64u * wave + 16u * ct + r16) * K + kk + kg);
 #pragma unroll
-    for (int rt = 0; rt < 4; rt++) a[rt] = *(const uint4 *)(act + (16u * rt + r16) * LDA + kk + kg);
+    for (int rt = 0; rt < RT; rt++) a[rt] = *(const uint4 *)(act + (16u * rt + r16) * LDA + kk + kg);
 #pragma unroll
-    for (int rt = 0; rt < 4; rt++)
+    for (int rt = 0; rt < RT; rt++)
 #pragma unroll
       for (int ct = 0; ct < 4; ct++) acc[rt][ct] = mma(a[rt], b[ct], acc[rt][ct]);
   }
@@ -171,12 +176,14 @@ template <class Src, bool TRAIN>
 __global__ __launch_bounds__(THREADS) void k_mlp_fwd(Src src, uint32_t n, Weights w, float *__restrict__ y,
                                                      float *__restrict__ Z, uint16_t *__restrict__ HT, uint32_t ldb)
 {
-  __shared__ __align__(16) uint16_t act[ROWS * LDA];
+  constexpr int RT = FWD_ROWS / 16;
+  __shared__ __align__(16) uint16_t act[FWD_ROWS * LDA];
   const uint32_t tid = threadIdx.x, lane = tid & 63u;
   const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const uint32_t row0 = blockIdx.x * ROWS;
-  {
-    const uint32_t r = tid >> 2, q = tid & 3u;
+  const uint32_t row0 = blockIdx.x * FWD_ROWS;
+#pragma unroll
+  for (uint32_t i = tid; i < FWD_ROWS * 4u; i += THREADS) {
+    const uint32_t r = i >> 2, q = i & 3u;
     const uint4 v = src.fill(row0 + r, q);
     *(uint4 *)(act + r * LDA + q * 8) = v;
     if (TRAIN) {
@@ -187,15 +194,15 @@ __global__ __launch_bounds__(THREADS) void k_mlp_fwd(Src src, uint32_t n, Weight
     }
   }
   __syncthreads();
-  f32x4 acc[4][4];
+  f32x4 acc[RT][4];
 #pragma unroll 1
   for (int layer = 0; layer < 3; layer++) {
     const float *bias = w.theta + (layer == 0 ? OFF_B1 : (layer == 1 ? OFF_B2 : OFF_B3));
-    if (layer == 0) tile_gemm<INP>(act, w.W1, lane, wave, acc);
-    else tile_gemm<HID>(act, layer == 1 ? w.W2 : w.W3, lane, wave, acc);
+    if (layer == 0) tile_gemm<INP, RT>(act, w.W1, lane, wave, acc);
+    else tile_gemm<HID, RT>(act, layer == 1 ? w.W2 : w.W3, lane, wave, acc);
     __syncthreads();   // every wave has read the layer input
 #pragma unroll
-    for (int rt = 0; rt < 4; rt++)
+    for (int rt = 0; rt < RT; rt++)
 #pragma unroll
       for (int ct = 0; ct < 4; ct++) {
         const uint32_t col = 64u * wave + 16u * ct + (lane & 15u);
@@ -219,13 +226,16 @@ __global__ __launch_bounds__(THREADS) void k_mlp_fwd(Src src, uint32_t n, Weight
     __syncthreads();
   }
   // output layer: y = h3 . w4 + b4 (FP32, fixed summation order)
-  const uint32_t r = tid >> 2, part = tid & 3u;
-  float s = 0.0f;
+#pragma unroll
+  for (uint32_t rr = 0; rr < FWD_ROWS; rr += THREADS / 4) {
+    const uint32_t r = rr + (tid >> 2), part = tid & 3u;
+    float s = 0.0f;
 #pragma unroll 8
-  for (int k = 0; k < 64; k++) s += bf2f(act[r * LDA + part * 64 + k]) * bf2f(w.w4[part * 64 + k]);
-  s += __shfl_xor(s, 1);
-  s += __shfl_xor(s, 2);
-  if (part == 0 && row0 + r < (TRAIN ? ldb : n)) y[row0 + r] = s + w.theta[OFF_B4];
+    for (int k = 0; k < 64; k++) s += bf2f(act[r * LDA + part * 64 + k]) * bf2f(w.w4[part * 64 + k]);
+    s += __shfl_xor(s, 1);
+    s += __shfl_xor(s, 2);
+    if (part == 0 && row0 + r < (TRAIN ? ldb : n)) y[row0 + r] = s + w.theta[OFF_B4];
+  }
 }
 
 // Backward of the mean squared error L = 1/n sum_r (y_r - t_r)^2 over the
