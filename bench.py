@@ -111,7 +111,11 @@ def extras():
                 one hipGraph launch (padenti_amd/predict_bench.py);
       proposal: learned split proposal vs exhaustive evaluation of a
                 20000-candidate pool on the same data
-                (padenti_amd/proposal_bench.py)."""
+                (padenti_amd/proposal_bench.py);
+      ingest:   uniform and class-weighted sampling of 5000 640x480 uint16
+                images resident in HBM (padenti_amd/ingest_bench.py), with
+                the CPU oracle (oracle/ingest.c, images in parallel over the
+                host cores) timed on the same images as its baseline."""
     res = {}
     try:
         from padenti_amd import predict_bench
@@ -132,7 +136,32 @@ def extras():
               file=sys.stderr, flush=True)
     except Exception as e:
         res["proposal"] = {"error": repr(e)}
+    try:
+        res["ingest"] = ingest_extra()
+    except Exception as e:
+        res["ingest"] = {"error": repr(e)}
     return res
+
+
+def ingest_extra(n_images=5000):
+    from padenti_amd import ingest_bench
+    from oracle import ingest as oi
+    from oracle import oracle as orc
+    a = ingest_bench.parser().parse_args(["--images", str(n_images)])
+    datas, labs = ingest_bench.make_dataset(a.images, a.w, a.h, a.classes, a.seed)
+    r = ingest_bench.run(a, datas, labs)
+    cpu = {}
+    for kind, name in ((0, "uniform"), (1, "class_weighted")):
+        t = time.perf_counter()
+        oi.sample_images(kind, datas, labs, a.samples, a.classes if kind else 0, seed=7)
+        dt = time.perf_counter() - t
+        cpu[name] = {"value": len(datas) / dt, "unit": "images/s", "cores": orc.num_threads(), "kind": "port",
+                     "sample": f"all {len(datas)} images, oracle/ingest.c (the reference's per-image loop), "
+                               "images in parallel"}
+        print(f"[bench] ingest {name}: GPU {r[name]['images_per_s']:.0f} images/s, CPU oracle "
+              f"{len(datas) / dt:.0f} images/s ({orc.num_threads()} threads)", file=sys.stderr, flush=True)
+    r["cpu_baseline"] = cpu
+    return r
 
 
 def main():
@@ -152,7 +181,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (trees or candidate features "
-              "would shard across GPUs, see DESIGN.md); run with --gpus 1", file=sys.stderr)
+              "would shard across GPUs); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import padenti_amd as pa

@@ -285,6 +285,92 @@ PD_API int pd_trainer_records_written(pd_trainer *tr, uint64_t *n);
 PD_API int pd_trainer_set_proposal(pd_trainer *tr, pd_mlp *mlp, uint32_t keep, uint32_t screen, uint32_t min_samples,
                                    int judge);
 
+/* ---- dataset ingestion ---------------------------------------------------
+ * Replaces (paths relative to the reference root):
+ *   UniformImageSampler::sample       src/padenti/uniform_image_sampler_impl.hpp:34-63
+ *   ClassWeightedImageSampler ctors    src/padenti/classweighted_image_sampler_impl.hpp:26-44
+ *   ClassWeightedImageSampler::sample  src/padenti/classweighted_image_sampler_impl.hpp:57-123
+ *   CVRGBLabelsLoader::load / _rgb2label  src/padenti/cv_image_loader_impl.hpp:78-133, 235-267
+ *   CVImageLoader::load (cv::imread)   src/padenti/cv_image_loader_impl.hpp:36-72
+ *   TrainingSet directory constructor  src/padenti/training_set_impl.hpp:43-133
+ * Sampling runs on the device, one workgroup per image, and reproduces the
+ * reference's draws: MT19937 seeded with crc32(image bytes) ^ seed. */
+#define PD_SAMPLER_UNIFORM 0        /* with replacement among labelled pixels */
+#define PD_SAMPLER_CLASS_WEIGHTED 1 /* per-class reservoir of ceilf(w_c * n_samples) */
+
+typedef struct {
+  uint32_t kind;         /* PD_SAMPLER_* */
+  uint32_t n_samples;    /* nSamples of the sampler constructor */
+  uint32_t seed;         /* seed of the sampler constructor (default 0) */
+  uint32_t n_classes;    /* class-weighted: nClasses (labels 1..n_classes) */
+  const float *weights;  /* class-weighted: n_classes weights, NULL = 1.f / n_classes each */
+} pd_sampler;
+
+typedef struct {
+  float crc_ms;    /* CRC-32 of the image bytes */
+  float index_ms;  /* uniform: labelled-pixel rank index (mask + prefix) */
+  float sample_ms; /* MT19937 draws and the reservoir / rank selection */
+  float sort_ms;   /* per-image ascending sort */
+  float total_ms;
+  uint64_t crc_bytes;   /* image bytes hashed */
+  uint64_t label_bytes; /* label bytes scanned */
+} pd_ingest_stats;
+
+/* Per-image output capacity (slots): n_samples (uniform) or sum_c
+ * ceilf(w_c * n_samples) (class-weighted; can exceed n_samples). */
+PD_API int pd_sampler_capacity(const pd_sampler *s, uint32_t *capacity);
+
+/* Sample a batch of images.  data: the image bytes (any layout; image i is
+ * the data_bytes[i] bytes at data_offsets[i], hashed as the reference's
+ * sizeof(type) * w * h * nChannels bytes); labels: uint8 label images
+ * (0 = unlabelled), image i at label_offsets[i], widths[i] * heights[i]
+ * bytes.  Output: samples[i * capacity + k], k < n_out[i], pixel indices
+ * y * w + x in ascending order; crc[i] (optional) = crc32 of image i.
+ * An image without labelled pixels gets n_out = 0 (the reference's draw
+ * range would be empty).  Class-weighted: a label above n_classes is an
+ * error (the reference indexes out of bounds). */
+PD_API int pd_sample_images(int device, const pd_sampler *s, uint32_t n_images, const void *data,
+                            const uint64_t *data_offsets, const uint64_t *data_bytes, const uint8_t *labels,
+                            const uint64_t *label_offsets, const uint32_t *widths, const uint32_t *heights,
+                            uint32_t *samples, uint32_t *n_out, uint32_t *crc);
+/* Device form: d_data, d_labels, d_samples, d_n_out, d_crc (optional) are
+ * device pointers; offsets, sizes and stats are host memory.  Every label
+ * image must start at a 16-byte aligned device address.  Synchronous. */
+PD_API int pd_sample_images_dev(int device, const pd_sampler *s, uint32_t n_images, const void *d_data,
+                                const uint64_t *data_offsets, const uint64_t *data_bytes, const uint8_t *d_labels,
+                                const uint64_t *label_offsets, const uint32_t *widths, const uint32_t *heights,
+                                uint32_t *d_samples, uint32_t *d_n_out, uint32_t *d_crc, pd_ingest_stats *stats);
+/* crc32 (zlib's) of n byte ranges of device memory; kernel_ms optional */
+PD_API int pd_crc32_images_dev(int device, uint32_t n, const void *d_data, const uint64_t *offsets,
+                               const uint64_t *bytes, uint32_t *d_crc, float *kernel_ms);
+/* labels[i] = 1 + first class c with rgb2label[c] == pixel i's (R, G, B),
+ * else 0; pixels are `channels` (3 or 4, the 4th ignored) bytes in R, G, B
+ * order (the reference converts OpenCV's BGR to RGB before matching). */
+PD_API int pd_rgb_to_labels(int device, uint64_t n_pixels, uint32_t channels, const uint8_t *rgb,
+                            const uint8_t *rgb2label, uint32_t n_classes, uint8_t *labels);
+PD_API int pd_rgb_to_labels_dev(int device, uint64_t n_pixels, uint32_t channels, const uint8_t *d_rgb,
+                                const uint8_t *rgb2label, uint32_t n_classes, uint8_t *d_labels, float *kernel_ms);
+
+/* PNG codec (host), standing in for cv::imread(path, -1) / cv::imwrite.
+ * Read: gray 1/2/4/8/16-bit -> 1 channel (low depths scaled to 8 bit);
+ * RGB, palette -> 3 channels B, G, R (4 with a tRNS chunk: B, G, R, A);
+ * gray + alpha, RGBA -> 4 channels B, G, R, A; 16-bit samples in host
+ * order.  pixels = NULL: header only.  capacity in bytes. */
+typedef struct {
+  uint32_t width, height, channels, bit_depth; /* bit_depth of the output: 8 or 16 */
+} pd_png_info;
+PD_API int pd_png_read(const char *path, pd_png_info *info, void *pixels, uint64_t capacity);
+/* channels 1 (gray), 3 (B, G, R) or 4 (B, G, R, A); bit_depth 8 or 16 */
+PD_API int pd_png_write(const char *path, uint32_t width, uint32_t height, uint32_t channels, uint32_t bit_depth,
+                        const void *pixels);
+/* Pairs of (dir/<p><data_suffix>, dir/<p><labels_suffix>) for every regular
+ * file of dir whose name ends with data_suffix and whose labels file
+ * exists, ordered by file name (the reference keeps the directory's own
+ * iteration order).  Written to buf as 2 * n_pairs NUL-terminated paths;
+ * *needed = bytes required (call with cap 0 to size buf). */
+PD_API int pd_list_pairs(const char *dir, const char *data_suffix, const char *labels_suffix, char *buf,
+                         uint64_t cap, uint32_t *n_pairs, uint64_t *needed);
+
 /* ---- device memory helpers (for callers without another allocator) ---- */
 PD_API int pd_dev_alloc(int device, uint64_t bytes, void **out);
 PD_API int pd_dev_free(void *p);

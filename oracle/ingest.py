@@ -1,0 +1,147 @@
+"""ctypes binding of the ingestion restatement (oracle/ingest.c, linked into
+liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/ as the checker and by the
+CPU-baseline leg of the ingestion benchmark, never by the product path.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import oracle as _o
+
+VP = C.c_void_p
+U32, U64 = C.c_uint32, C.c_uint64
+_ready = False
+
+
+def lib():
+    global _ready
+    L = _o.lib()
+    if not _ready:
+        L.or_crc32.restype = U32
+        L.or_crc32.argtypes = [VP, U64]
+        L.or_mt_outputs.argtypes = [U32, U64, VP]
+        L.or_sample_uniform.restype = U32
+        L.or_sample_uniform.argtypes = [VP, U64, VP, U32, U32, U32, U32, VP]
+        L.or_sample_uniform_stream.restype = U32
+        L.or_sample_uniform_stream.argtypes = [VP, U32, U32, U32, VP, U64, VP, C.POINTER(U64)]
+        L.or_bucket_sizes.argtypes = [U32, U32, VP, VP]
+        L.or_sample_weighted.restype = U32
+        L.or_sample_weighted.argtypes = [VP, U64, VP, U32, U32, U32, U32, VP, U32, VP]
+        L.or_sample_weighted_stream.restype = U32
+        L.or_sample_weighted_stream.argtypes = [VP, U32, U32, U32, U32, VP, VP, U64, VP, C.POINTER(U64)]
+        L.or_sample_images.argtypes = [C.c_int, U32, VP, VP, VP, VP, VP, VP, VP, U32, U32, VP, U32, U32, VP, VP, VP]
+        L.or_rgb_to_labels.argtypes = [VP, U64, U32, VP, U32, VP]
+        _ready = True
+    return L
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def _bytes(a):
+    return np.ascontiguousarray(a).view(np.uint8).reshape(-1)
+
+
+def crc32(data):
+    b = _bytes(data)
+    return int(lib().or_crc32(_p(b), b.size))
+
+
+def mt_outputs(seed, n):
+    out = np.zeros(n, dtype=np.uint32)
+    lib().or_mt_outputs(int(seed) & 0xFFFFFFFF, n, _p(out))
+    return out
+
+
+def bucket_sizes(n_samples, n_classes, weights=None):
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float32)
+    out = np.zeros(n_classes, dtype=np.uint32)
+    lib().or_bucket_sizes(n_samples, n_classes, _p(w), _p(out))
+    return out
+
+
+def sample_uniform(data, labels, n_samples, seed=0):
+    """UniformImageSampler(n_samples, seed).sample(data, labels, w, h)."""
+    lab = np.ascontiguousarray(labels, dtype=np.uint8)
+    h, w = lab.shape
+    b = _bytes(data)
+    out = np.zeros(max(n_samples, 1), dtype=np.uint32)
+    n = lib().or_sample_uniform(_p(b), b.size, _p(lab), w, h, n_samples, int(seed) & 0xFFFFFFFF, _p(out))
+    return out[:n]
+
+
+def sample_weighted(data, labels, n_samples, n_classes, weights=None, seed=0):
+    """ClassWeightedImageSampler(n_samples, n_classes[, weights], seed).sample(...)."""
+    lab = np.ascontiguousarray(labels, dtype=np.uint8)
+    h, w = lab.shape
+    b = _bytes(data)
+    cap = int(bucket_sizes(n_samples, n_classes, weights).sum())
+    wt = None if weights is None else np.ascontiguousarray(weights, dtype=np.float32)
+    out = np.zeros(max(cap, 1), dtype=np.uint32)
+    n = lib().or_sample_weighted(_p(b), b.size, _p(lab), w, h, n_samples, n_classes, _p(wt),
+                                 int(seed) & 0xFFFFFFFF, _p(out))
+    return out[:n]
+
+
+def sample_uniform_stream(labels, n_samples, stream):
+    """The uniform sampler drawing from the given generator outputs; returns
+    (samples, outputs used) (used = -1 when the stream ran out)."""
+    lab = np.ascontiguousarray(labels, dtype=np.uint8)
+    h, w = lab.shape
+    s = np.ascontiguousarray(stream, dtype=np.uint32)
+    out = np.zeros(max(n_samples, 1), dtype=np.uint32)
+    used = U64(0)
+    n = lib().or_sample_uniform_stream(_p(lab), w, h, n_samples, _p(s), s.size, _p(out), C.byref(used))
+    u = used.value
+    return out[:n], (-1 if u == 2 ** 64 - 1 else u)
+
+
+def sample_weighted_stream(labels, n_samples, n_classes, stream, weights=None):
+    lab = np.ascontiguousarray(labels, dtype=np.uint8)
+    h, w = lab.shape
+    s = np.ascontiguousarray(stream, dtype=np.uint32)
+    cap = int(bucket_sizes(n_samples, n_classes, weights).sum())
+    wt = None if weights is None else np.ascontiguousarray(weights, dtype=np.float32)
+    out = np.zeros(max(cap, 1), dtype=np.uint32)
+    used = U64(0)
+    n = lib().or_sample_weighted_stream(_p(lab), w, h, n_samples, n_classes, _p(wt), _p(s), s.size, _p(out),
+                                        C.byref(used))
+    u = used.value
+    return out[:n], (-1 if u == 2 ** 64 - 1 else u)
+
+
+def sample_images(kind, images, labels, n_samples, n_classes=0, weights=None, seed=0):
+    """Batch over images (OpenMP); kind 0 uniform, 1 class-weighted.
+    Returns (list of sample arrays, crc array)."""
+    n = len(images)
+    datas = [_bytes(d) for d in images]
+    labs = [np.ascontiguousarray(l, dtype=np.uint8) for l in labels]
+    data = np.concatenate(datas) if n else np.zeros(0, np.uint8)
+    lab = np.concatenate([l.reshape(-1) for l in labs]) if n else np.zeros(0, np.uint8)
+    dbytes = np.array([d.size for d in datas], dtype=np.uint64)
+    doff = np.concatenate([[0], np.cumsum(dbytes)[:-1]]).astype(np.uint64)
+    lbytes = np.array([l.size for l in labs], dtype=np.uint64)
+    loff = np.concatenate([[0], np.cumsum(lbytes)[:-1]]).astype(np.uint64)
+    widths = np.array([l.shape[1] for l in labs], dtype=np.uint32)
+    heights = np.array([l.shape[0] for l in labs], dtype=np.uint32)
+    cap = n_samples if kind == 0 else int(bucket_sizes(n_samples, n_classes, weights).sum())
+    out = np.zeros(max(n * cap, 1), dtype=np.uint32)
+    n_out = np.zeros(n, dtype=np.uint32)
+    crc = np.zeros(n, dtype=np.uint32)
+    wt = None if weights is None else np.ascontiguousarray(weights, dtype=np.float32)
+    lib().or_sample_images(kind, n, _p(data), _p(doff), _p(dbytes), _p(lab), _p(loff), _p(widths), _p(heights),
+                           n_samples, n_classes, _p(wt), int(seed) & 0xFFFFFFFF, cap, _p(out), _p(n_out), _p(crc))
+    return [out[i * cap:i * cap + n_out[i]].copy() for i in range(n)], crc
+
+
+def rgb_to_labels(rgb, rgb2label):
+    """_rgb2label on an (h, w, 3|4) R, G, B(, A) array."""
+    a = np.ascontiguousarray(rgb, dtype=np.uint8)
+    ch = a.shape[-1]
+    m = np.ascontiguousarray(rgb2label, dtype=np.uint8).reshape(-1, 3)
+    out = np.zeros(a.shape[:-1], dtype=np.uint8)
+    lib().or_rgb_to_labels(_p(a), out.size, ch, _p(m), m.shape[0], _p(out))
+    return out

@@ -25,6 +25,9 @@ PD_SYN_CAPSULES_U16 = 1
 PD_SYN_CAPSULES_RGBD = 2
 PD_SYN_BLOBS_U16 = 3
 
+PD_SAMPLER_UNIFORM = 0
+PD_SAMPLER_CLASS_WEIGHTED = 1
+
 # (image dtype, channels, feature dtype, feature dim) of each compiled hook
 FEATURES = {
     PD_FEAT_DEPTH_F32: (np.float32, 1, np.float32, 4),
@@ -94,6 +97,20 @@ class TuningRule(C.Structure):
 
 class Adam(C.Structure):
     _fields_ = [("lr", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float)]
+
+
+class Sampler(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("n_samples", C.c_uint32), ("seed", C.c_uint32), ("n_classes", C.c_uint32),
+                ("weights", C.c_void_p)]
+
+
+class IngestStats(C.Structure):
+    _fields_ = [("crc_ms", C.c_float), ("index_ms", C.c_float), ("sample_ms", C.c_float), ("sort_ms", C.c_float),
+                ("total_ms", C.c_float), ("crc_bytes", C.c_uint64), ("label_bytes", C.c_uint64)]
+
+
+class PngInfo(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("channels", C.c_uint32), ("bit_depth", C.c_uint32)]
 
 
 _lib = None
@@ -167,6 +184,16 @@ def lib():
         "pd_memcpy_h2d": (I32, [VP, VP, U64]),
         "pd_memcpy_d2h": (I32, [VP, VP, U64]),
         "pd_dev_sync": (I32, [I32]),
+        "pd_sampler_capacity": (I32, [C.POINTER(Sampler), C.POINTER(U32)]),
+        "pd_sample_images": (I32, [I32, C.POINTER(Sampler), U32, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP]),
+        "pd_sample_images_dev": (I32, [I32, C.POINTER(Sampler), U32, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP,
+                                       C.POINTER(IngestStats)]),
+        "pd_crc32_images_dev": (I32, [I32, U32, VP, VP, VP, VP, C.POINTER(C.c_float)]),
+        "pd_rgb_to_labels": (I32, [I32, U64, U32, VP, VP, U32, VP]),
+        "pd_rgb_to_labels_dev": (I32, [I32, U64, U32, VP, VP, U32, VP, C.POINTER(C.c_float)]),
+        "pd_png_read": (I32, [C.c_char_p, C.POINTER(PngInfo), VP, U64]),
+        "pd_png_write": (I32, [C.c_char_p, U32, U32, U32, U32, VP]),
+        "pd_list_pairs": (I32, [C.c_char_p, C.c_char_p, C.c_char_p, VP, U64, C.POINTER(U32), C.POINTER(U64)]),
         "pd_synth_image": (I32, [I32, U64, U32, U32, U32, U32, VP, VP]),
         "pd_synth_samples": (I32, [U64, U32, VP, U32, U32, U32, VP]),
     }

@@ -16,7 +16,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libpadenti_hip.so")
-SOURCES = [os.path.join(HERE, "csrc", "pd_capi.hip"), os.path.join(HERE, "csrc", "pd_synth.cpp")]
+SOURCES = [os.path.join(HERE, "csrc", n) for n in ("pd_capi.hip", "pd_ingest.hip", "pd_synth.cpp", "pd_png.cpp")]
+LIBS = ["-lz"]  # PNG streams
 # example of a separately compiled feature hook (include/padenti/pd_plugin.hpp)
 PLUGIN_LIB = os.path.join(HERE, "libpd_example_plugin.so")
 PLUGIN_SOURCES = [os.path.join(HERE, "examples", "depth_plugin.hip")]
@@ -38,12 +39,24 @@ HIP_FLAGS = [
 ]
 
 
-def _compile(sources, out, verbose):
-    tmp = out + ".tmp"
-    cmd = [HIPCC] + HIP_FLAGS + sources + ["-o", tmp]
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
+
+
+def _compile(sources, out, verbose, libs=()):
+    """Objects side by side (one hipcc per source), then one link."""
+    from concurrent.futures import ThreadPoolExecutor
+    objdir = os.path.join(HERE, "_obj")
+    os.makedirs(objdir, exist_ok=True)
+    flags = [f for f in HIP_FLAGS if f != "-shared"]
+    objs = [os.path.join(objdir, os.path.basename(out) + "." + os.path.basename(s) + ".o") for s in sources]
+    with ThreadPoolExecutor(len(sources)) as ex:
+        for j in [ex.submit(_run, [HIPCC] + flags + ["-c", s, "-o", o], verbose) for s, o in zip(sources, objs)]:
+            j.result()
+    tmp = out + ".tmp"
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC"] + objs + list(libs) + ["-o", tmp], verbose)
     os.replace(tmp, out)
 
 
@@ -67,12 +80,12 @@ def build(verbose: bool = False, plugin: bool = True) -> str:
         # the two libraries compile independently: run them side by side
         from concurrent.futures import ThreadPoolExecutor
         with ThreadPoolExecutor(2) as ex:
-            jobs = [ex.submit(_compile, SOURCES, LIB, verbose), ex.submit(_compile, PLUGIN_SOURCES, PLUGIN_LIB, verbose)]
+            jobs = [ex.submit(_compile, SOURCES, LIB, verbose, LIBS), ex.submit(_compile, PLUGIN_SOURCES, PLUGIN_LIB, verbose)]
             for j in jobs:
                 j.result()
         build_demo(verbose)
     else:
-        _compile(SOURCES, LIB, verbose)
+        _compile(SOURCES, LIB, verbose, LIBS)
     return LIB
 
 

@@ -1,7 +1,9 @@
 // The C++ drop-in API used the way the reference's demo uses CLTreeTrainer /
 // CLClassifier (test/test_tree_trainer.cpp:53-98, test/test_classifier.cpp):
 // build a training set, train trees with a built-in hook and with a plugin
-// hook, save / load them as XML, classify a held-out image.  Writes the
+// hook, save / load them as XML, classify a held-out image; build a training
+// set from a directory of PNGs (test/test_training_set.cpp:34-39) with both
+// samplers.  Writes the
 // arrays it produced to <outdir> for the test-suite to compare with the CPU
 // oracle.   usage: api_demo <outdir>
 #include <cstdio>
@@ -10,6 +12,8 @@
 #include <fstream>
 #include <string>
 #include <vector>
+
+#include <sys/stat.h>
 
 #include <padenti/padenti.hpp>
 
@@ -170,6 +174,48 @@ int main(int argc, char **argv)
     dump(out + "/leaves.bin", leaves.getData(), (size_t)W * H * 4);
   } catch (const std::exception &e) {
     fprintf(stderr, "error: %s\n", e.what());
+    return 1;
+  }
+  try {  // a training set from a directory of depth / RGB-label PNGs
+    static const unsigned char RGB2LABEL[8][3] = {{255, 0, 0}, {0, 0, 255}, {0, 255, 0}, {255, 255, 0},
+                                                  {255, 0, 255}, {0, 255, 255}, {128, 64, 32}, {250, 250, 250}};
+    const std::string dir = out + "/ds";
+    mkdir(dir.c_str(), 0755);
+    const unsigned DW = 96, DH = 64, ND = 5;
+    std::vector<unsigned short> depth(DW * DH);
+    std::vector<uint8_t> dl(DW * DH), bgr(DW * DH * 3);
+    for (unsigned i = 0; i < ND; i++) {
+      pd_synth_image(1, 0, i, DW, DH, 8, depth.data(), dl.data());
+      for (unsigned k = 0; k < DW * DH; k++)
+        for (unsigned c = 0; c < 3; c++) bgr[3 * k + c] = dl[k] ? RGB2LABEL[dl[k] - 1][2 - c] : 0;
+      char name[64];
+      snprintf(name, sizeof(name), "/s%02u_depth.png", i);
+      pd_cxx::io_check(pd_png_write((dir + name).c_str(), DW, DH, 1, 16, depth.data()));
+      snprintf(name, sizeof(name), "/s%02u_labels.png", i);
+      pd_cxx::io_check(pd_png_write((dir + name).c_str(), DW, DH, 3, 8, bgr.data()));
+    }
+    PNGImageLoader<unsigned short, 1> depthLoader;
+    RGBLabelsLoader labelsLoader(RGB2LABEL, 8);
+    UniformImageSampler<unsigned short, 1> uniform(300, 7);
+    ClassWeightedImageSampler<unsigned short, 1> weighted(300, 8, 7);
+    TrainingSet<unsigned short, 1> tsu(dir, "_depth.png", "_labels.png", 8, depthLoader, labelsLoader, uniform);
+    TrainingSet<unsigned short, 1> tsw(dir, "_depth.png", "_labels.png", 8, depthLoader, labelsLoader, weighted);
+    if (tsu.getNImages() != ND || tsw.getNImages() != ND) {
+      fprintf(stderr, "directory training set has %u / %u images\n", tsu.getNImages(), tsw.getNImages());
+      return 1;
+    }
+    std::vector<uint32_t> counts, all;
+    for (unsigned i = 0; i < ND; i++)
+      for (int k = 0; k < 2; k++) {
+        const TrainingSetImage<unsigned short, 1> &im = (k ? tsw : tsu).getImages()[i];
+        counts.push_back(im.getNSamples());
+        all.insert(all.end(), im.getSamples(), im.getSamples() + im.getNSamples());
+      }
+    dump(out + "/ds_counts.bin", counts.data(), counts.size() * 4);
+    dump(out + "/ds_samples.bin", all.data(), all.size() * 4);
+    dump(out + "/ds_priors.bin", tsu.getPriors(), 8 * 4);
+  } catch (const std::exception &e) {
+    fprintf(stderr, "dataset error: %s\n", e.what());
     return 1;
   }
   printf("api_demo ok\n");

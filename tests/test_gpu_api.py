@@ -37,6 +37,18 @@ def test_cpp_api_demo_matches_oracle(gpu, tmp_path):
     mask = (np.arange(160 * 120) % 3 != 0).astype(np.uint8).reshape(120, 160)
     assert rd("post_masked.bin", np.float32).tobytes() == orc.predict_posterior(0, img, mask, [tree], 4).tobytes()
     assert np.array_equal(rd("leaves.bin", np.int32).reshape(120, 160), orc.predict_leaves(0, img, None, tree))
+    # the directory-built training sets (uniform and class-weighted samplers)
+    from oracle import ingest as oi
+    counts, allv = rd("ds_counts.bin", np.uint32), rd("ds_samples.bin", np.uint32)
+    o, acc = 0, np.zeros(8)
+    for i in range(5):
+        depth, lab = synth.gen_image(pa._lib.PD_SYN_CAPSULES_U16, 0, i, 96, 64, 8)
+        want = [oi.sample_uniform(depth, lab, 300, seed=7), oi.sample_weighted(depth, lab, 300, 8, seed=7)]
+        for k in range(2):
+            assert np.array_equal(allv[o:o + counts[2 * i + k]], want[k]), (i, k)
+            o += counts[2 * i + k]
+        acc += pa.TrainingSetImage(depth, 96, 64, lab, 8, want[0]).priors
+    assert rd("ds_priors.bin", np.float32).tobytes() == (acc / 5).astype(np.float32).tobytes()
 
 
 def test_plugin_hook_trains_oracle_trees(gpu):
