@@ -333,16 +333,20 @@ PD_API int pd_sample_images(int device, const pd_sampler *s, uint32_t n_images, 
                             const uint64_t *data_offsets, const uint64_t *data_bytes, const uint8_t *labels,
                             const uint64_t *label_offsets, const uint32_t *widths, const uint32_t *heights,
                             uint32_t *samples, uint32_t *n_out, uint32_t *crc);
-/* Device form: d_data, d_labels, d_samples, d_n_out, d_crc (optional) are
- * device pointers; offsets, sizes and stats are host memory.  Every label
- * image must start at a 16-byte aligned device address.  Synchronous. */
+/* Device form: d_data (data_size bytes), d_labels (labels_size bytes),
+ * d_samples (n_images * capacity slots), d_n_out, d_crc (optional, n_images)
+ * are device pointers; offsets, sizes and stats are host memory.  Every
+ * range must lie inside its buffer and every label image must start at a
+ * 16-byte aligned device address (both checked).  Synchronous. */
 PD_API int pd_sample_images_dev(int device, const pd_sampler *s, uint32_t n_images, const void *d_data,
-                                const uint64_t *data_offsets, const uint64_t *data_bytes, const uint8_t *d_labels,
-                                const uint64_t *label_offsets, const uint32_t *widths, const uint32_t *heights,
-                                uint32_t *d_samples, uint32_t *d_n_out, uint32_t *d_crc, pd_ingest_stats *stats);
-/* crc32 (zlib's) of n byte ranges of device memory; kernel_ms optional */
-PD_API int pd_crc32_images_dev(int device, uint32_t n, const void *d_data, const uint64_t *offsets,
-                               const uint64_t *bytes, uint32_t *d_crc, float *kernel_ms);
+                                uint64_t data_size, const uint64_t *data_offsets, const uint64_t *data_bytes,
+                                const uint8_t *d_labels, uint64_t labels_size, const uint64_t *label_offsets,
+                                const uint32_t *widths, const uint32_t *heights, uint32_t *d_samples,
+                                uint32_t *d_n_out, uint32_t *d_crc, pd_ingest_stats *stats);
+/* crc32 (zlib's) of n byte ranges of device memory d_data (data_size bytes;
+ * every range is checked against it); kernel_ms optional */
+PD_API int pd_crc32_images_dev(int device, uint32_t n, const void *d_data, uint64_t data_size,
+                               const uint64_t *offsets, const uint64_t *bytes, uint32_t *d_crc, float *kernel_ms);
 /* labels[i] = 1 + first class c with rgb2label[c] == pixel i's (R, G, B),
  * else 0; pixels are `channels` (3 or 4, the 4th ignored) bytes in R, G, B
  * order (the reference converts OpenCV's BGR to RGB before matching). */

@@ -186,9 +186,9 @@ def lib():
         "pd_dev_sync": (I32, [I32]),
         "pd_sampler_capacity": (I32, [C.POINTER(Sampler), C.POINTER(U32)]),
         "pd_sample_images": (I32, [I32, C.POINTER(Sampler), U32, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP]),
-        "pd_sample_images_dev": (I32, [I32, C.POINTER(Sampler), U32, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP,
-                                       C.POINTER(IngestStats)]),
-        "pd_crc32_images_dev": (I32, [I32, U32, VP, VP, VP, VP, C.POINTER(C.c_float)]),
+        "pd_sample_images_dev": (I32, [I32, C.POINTER(Sampler), U32, VP, U64, VP, VP, VP, U64, VP, VP, VP, VP, VP,
+                                       VP, C.POINTER(IngestStats)]),
+        "pd_crc32_images_dev": (I32, [I32, U32, VP, U64, VP, VP, VP, C.POINTER(C.c_float)]),
         "pd_rgb_to_labels": (I32, [I32, U64, U32, VP, VP, U32, VP]),
         "pd_rgb_to_labels_dev": (I32, [I32, U64, U32, VP, VP, U32, VP, C.POINTER(C.c_float)]),
         "pd_png_read": (I32, [C.c_char_p, C.POINTER(PngInfo), VP, U64]),

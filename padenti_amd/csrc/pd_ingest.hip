@@ -91,8 +91,17 @@ uint32_t capacity_of(const pd_sampler &s)
   return (uint32_t)t;
 }
 
-// CRC jobs of n byte ranges of `base`; launches zero-fill, the block kernel
-// and the finalisation on the context stream, crc -> d_crc.
+// every [off, off + bytes) inside a buffer of `size` bytes
+void check_ranges(uint32_t n, const uint64_t *off, const uint64_t *bytes, uint64_t size, const char *what)
+{
+  for (uint32_t i = 0; i < n; i++)
+    require(off[i] <= size && bytes[i] <= size - off[i],
+            std::string(what) + " range " + std::to_string(i) + " lies outside its buffer");
+}
+
+// CRC jobs of n byte ranges of `base` (checked by the caller); launches
+// zero-fill, the block kernel and the finalisation on the context stream,
+// crc -> d_crc.
 void launch_crc(IngestDev &X, uint32_t n, const uint8_t *base, const uint64_t *off, const uint64_t *bytes,
                 uint32_t *d_crc)
 {
@@ -138,14 +147,20 @@ struct Timer {
   float ms(int a, int b) const { return (float)elapsed_ms(ev[a], ev[b]); }
 };
 
-void sample_dev(int device, const pd_sampler &s, uint32_t n, const void *d_data, const uint64_t *data_off,
-                const uint64_t *data_bytes, const uint8_t *d_labels, const uint64_t *lab_off, const uint32_t *widths,
-                const uint32_t *heights, uint32_t *d_samples, uint32_t *d_n_out, uint32_t *d_crc,
-                pd_ingest_stats *stats)
+void sample_dev(int device, const pd_sampler &s, uint32_t n, const void *d_data, uint64_t data_size,
+                const uint64_t *data_off, const uint64_t *data_bytes, const uint8_t *d_labels, uint64_t labels_size,
+                const uint64_t *lab_off, const uint32_t *widths, const uint32_t *heights, uint32_t *d_samples,
+                uint32_t *d_n_out, uint32_t *d_crc, pd_ingest_stats *stats)
 {
   require(n >= 1, "n_images must be >= 1");
   require(d_data && data_off && data_bytes && d_labels && lab_off && widths && heights && d_samples && d_n_out,
           "null pointer in sample_images");
+  check_ranges(n, data_off, data_bytes, data_size, "image data");
+  {
+    std::vector<uint64_t> lb(n);
+    for (uint32_t i = 0; i < n; i++) lb[i] = (uint64_t)widths[i] * heights[i];
+    check_ranges(n, lab_off, lb.data(), labels_size, "label image");
+  }
   const uint32_t cap = capacity_of(s);
   require((uint64_t)n * cap < (1ull << 31), "n_images * capacity must be < 2^31");
   set_device(device);
@@ -311,15 +326,16 @@ int pd_sampler_capacity(const pd_sampler *s, uint32_t *capacity)
   });
 }
 
-int pd_sample_images_dev(int device, const pd_sampler *s, uint32_t n_images, const void *d_data,
+int pd_sample_images_dev(int device, const pd_sampler *s, uint32_t n_images, const void *d_data, uint64_t data_size,
                          const uint64_t *data_offsets, const uint64_t *data_bytes, const uint8_t *d_labels,
-                         const uint64_t *label_offsets, const uint32_t *widths, const uint32_t *heights,
-                         uint32_t *d_samples, uint32_t *d_n_out, uint32_t *d_crc, pd_ingest_stats *stats)
+                         uint64_t labels_size, const uint64_t *label_offsets, const uint32_t *widths,
+                         const uint32_t *heights, uint32_t *d_samples, uint32_t *d_n_out, uint32_t *d_crc,
+                         pd_ingest_stats *stats)
 {
   return guarded([&] {
     require(s != nullptr, "null sampler");
-    sample_dev(device, *s, n_images, d_data, data_offsets, data_bytes, d_labels, label_offsets, widths, heights,
-               d_samples, d_n_out, d_crc, stats);
+    sample_dev(device, *s, n_images, d_data, data_size, data_offsets, data_bytes, d_labels, labels_size,
+               label_offsets, widths, heights, d_samples, d_n_out, d_crc, stats);
   });
 }
 
@@ -346,19 +362,20 @@ int pd_sample_images(int device, const pd_sampler *s, uint32_t n_images, const v
     X.n_out.ensure((size_t)n_images * 4u);
     X.crc_out.ensure((size_t)n_images * 4u);
     uint32_t *ds = X.samples.as<uint32_t>(), *dn = X.n_out.as<uint32_t>(), *dc = X.crc_out.as<uint32_t>();
-    sample_dev(device, *s, n_images, dd, doff.data(), data_bytes, dl, loff.data(), widths, heights, ds, dn, dc,
-               nullptr);
+    sample_dev(device, *s, n_images, dd, X.data.cap, doff.data(), data_bytes, dl, X.labels.cap, loff.data(), widths,
+               heights, ds, dn, dc, nullptr);
     HIPCHK(hipMemcpy(samples, ds, (size_t)n_images * cap * 4u, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(n_out, dn, (size_t)n_images * 4u, hipMemcpyDeviceToHost));
     if (crc) HIPCHK(hipMemcpy(crc, dc, (size_t)n_images * 4u, hipMemcpyDeviceToHost));
   });
 }
 
-int pd_crc32_images_dev(int device, uint32_t n, const void *d_data, const uint64_t *offsets, const uint64_t *bytes,
-                        uint32_t *d_crc, float *kernel_ms)
+int pd_crc32_images_dev(int device, uint32_t n, const void *d_data, uint64_t data_size, const uint64_t *offsets,
+                        const uint64_t *bytes, uint32_t *d_crc, float *kernel_ms)
 {
   return guarded([&] {
     require(n >= 1 && d_data && offsets && bytes && d_crc, "bad CRC arguments");
+    check_ranges(n, offsets, bytes, data_size, "CRC");
     set_device(device);
     IngestDev &X = ctx(device);
     Lock g(X.mu);

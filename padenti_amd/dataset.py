@@ -151,14 +151,23 @@ class _Sampler:
         res = [out[i * cap:i * cap + n_out[i]].copy() for i in range(n)]
         return (res, crcs) if crc else res
 
-    def sample_dev(self, n, d_data, data_off, data_bytes, d_labels, lab_off, widths, heights, d_samples, d_n_out,
-                   d_crc=None):
-        """Device-resident form (pd_sample_images_dev); returns the stage timings."""
+    def sample_dev(self, n, d_data, data_size, data_off, data_bytes, d_labels, labels_size, lab_off, widths, heights,
+                   d_samples, d_n_out, d_crc=None):
+        """Device-resident form (pd_sample_images_dev): d_* are device
+        pointers, data_size / labels_size the bytes of those buffers, the
+        rest host arrays (uint64 offsets and byte counts, uint32 sizes).
+        Returns the stage timings."""
         st = IngestStats()
         desc = self._desc()
-        check(lib().pd_sample_images_dev(self.device, C.byref(desc), n, d_data, ptr(data_off), ptr(data_bytes),
-                                         d_labels, ptr(lab_off), ptr(widths), ptr(heights), d_samples, d_n_out,
-                                         d_crc, C.byref(st)))
+        arrs = [np.ascontiguousarray(a, dtype=t) for a, t in ((data_off, np.uint64), (data_bytes, np.uint64),
+                                                             (lab_off, np.uint64), (widths, np.uint32),
+                                                             (heights, np.uint32))]
+        for a in arrs:
+            if a.size != n:
+                raise ValueError("one offset / size per image")
+        check(lib().pd_sample_images_dev(self.device, C.byref(desc), n, d_data, int(data_size), ptr(arrs[0]),
+                                         ptr(arrs[1]), d_labels, int(labels_size), ptr(arrs[2]), ptr(arrs[3]),
+                                         ptr(arrs[4]), d_samples, d_n_out, d_crc, C.byref(st)))
         self.last_stats = {f: getattr(st, f) for f, _ in IngestStats._fields_}
         return self.last_stats
 

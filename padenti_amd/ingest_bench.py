@@ -71,9 +71,13 @@ def run(a, datas=None, labs=None):
     npx = a.w * a.h
     img_bytes = npx * 2
     dd, dl = _Dev(n * img_bytes), _Dev(n * npx)
-    # one copy each (images and label images packed back to back; 640x480 keeps 16-byte alignment)
-    L.check(L.lib().pd_memcpy_h2d(dd.p, L.ptr(np.ascontiguousarray(np.stack(datas))), n * img_bytes))
-    L.check(L.lib().pd_memcpy_h2d(dl.p, L.ptr(np.ascontiguousarray(np.stack(labs))), n * npx))
+    # one copy each (images and label images packed back to back; 640x480 keeps 16-byte alignment);
+    # the packed host arrays stay referenced until the copies return
+    packed = np.ascontiguousarray(np.stack(datas))
+    L.check(L.lib().pd_memcpy_h2d(dd.p, L.ptr(packed), n * img_bytes))
+    packed = np.ascontiguousarray(np.stack(labs))
+    L.check(L.lib().pd_memcpy_h2d(dl.p, L.ptr(packed), n * npx))
+    del packed
     doff = (np.arange(n, dtype=np.uint64) * img_bytes).astype(np.uint64)
     dbytes = np.full(n, img_bytes, np.uint64)
     loff = (np.arange(n, dtype=np.uint64) * npx).astype(np.uint64)
@@ -85,11 +89,12 @@ def run(a, datas=None, labs=None):
                       ("class_weighted", ds.ClassWeightedImageSampler(a.samples, a.classes, seed=7))):
         cap = smp.capacity()
         ds_out, dn, dc = _Dev(4 * n * cap), _Dev(4 * n), _Dev(4 * n)
-        smp.sample_dev(n, dd.p, doff, dbytes, dl.p, loff, w, h, ds_out.p, dn.p, dc.p)   # warm-up
+        args = (n, dd.p, n * img_bytes, doff, dbytes, dl.p, n * npx, loff, w, h, ds_out.p, dn.p, dc.p)
+        smp.sample_dev(*args)   # warm-up
         runs = []
         for _ in range(a.reps):
             t = time.perf_counter()
-            st = smp.sample_dev(n, dd.p, doff, dbytes, dl.p, loff, w, h, ds_out.p, dn.p, dc.p)
+            st = smp.sample_dev(*args)
             st["wall_ms"] = (time.perf_counter() - t) * 1e3
             runs.append(st)
         med = {k: float(np.median([r[k] for r in runs])) for k in ("crc_ms", "index_ms", "sample_ms", "sort_ms",

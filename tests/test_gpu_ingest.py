@@ -49,8 +49,9 @@ def test_crc32_ragged_ranges(gpu):
     d.put(buf)
     crc = DevArray(4 * len(lengths))
     ms = C.c_float()
-    check(lib().pd_crc32_images_dev(0, len(lengths), d.p, ptr(np.array(offs, np.uint64)),
-                                    ptr(np.array(lengths, np.uint64)), crc.p, C.byref(ms)))
+    offs_a, len_a = np.array(offs, np.uint64), np.array(lengths, np.uint64)   # alive across the call
+    assert int((offs_a + len_a).max()) <= buf.size
+    check(lib().pd_crc32_images_dev(0, len(lengths), d.p, buf.size, ptr(offs_a), ptr(len_a), crc.p, C.byref(ms)))
     got = crc.get(len(lengths), np.uint32)
     for i, (s, n) in enumerate(zip(offs, lengths)):
         assert got[i] == zlib.crc32(buf[s:s + n].tobytes()), (i, s, n)
@@ -141,10 +142,11 @@ def test_device_entry_point_and_stats(gpu):
         dd.put(r, int(o))
     for l, o in zip(labs, loff):
         dl.put(l, int(o))
+    dsize, lsize = int(doff[-1] + dbytes[-1]), int(loff[-1] + lsz[-1])
     for smp in (ds.UniformImageSampler(300, seed=2), ds.ClassWeightedImageSampler(300, 8, seed=2)):
         cap = smp.capacity()
         out, nout, crc = DevArray(4 * cap * len(labs)), DevArray(4 * len(labs)), DevArray(4 * len(labs))
-        st = smp.sample_dev(len(labs), dd.p, doff, dbytes, dl.p, loff, w, h, out.p, nout.p, crc.p)
+        st = smp.sample_dev(len(labs), dd.p, dsize, doff, dbytes, dl.p, lsize, loff, w, h, out.p, nout.p, crc.p)
         assert st["crc_bytes"] == int(dbytes.sum()) and st["label_bytes"] == int(lsz.sum())
         assert st["total_ms"] > 0
         n_out = nout.get(len(labs), np.uint32)
@@ -158,7 +160,15 @@ def test_device_entry_point_and_stats(gpu):
     bad = loff.copy()
     bad[1] += 1
     with pytest.raises(pa.PdError, match="16-byte aligned"):
-        ds.UniformImageSampler(10).sample_dev(len(labs), dd.p, doff, dbytes, dl.p, bad, w, h, dd.p, dd.p)
+        ds.UniformImageSampler(10).sample_dev(len(labs), dd.p, dsize, doff, dbytes, dl.p, lsize, bad, w, h, dd.p,
+                                              dd.p)
+    # ranges are checked against the buffer sizes before any launch
+    with pytest.raises(pa.PdError, match="outside its buffer"):
+        ds.UniformImageSampler(10).sample_dev(len(labs), dd.p, dsize - 1, doff, dbytes, dl.p, lsize, loff, w, h,
+                                              dd.p, dd.p)
+    with pytest.raises(pa.PdError, match="outside its buffer"):
+        ds.UniformImageSampler(10).sample_dev(len(labs), dd.p, dsize, doff, dbytes, dl.p, lsize - 16, loff, w, h,
+                                              dd.p, dd.p)
     dd.free()
     dl.free()
 

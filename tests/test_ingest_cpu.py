@@ -138,3 +138,23 @@ def test_loader_type_checks(tmp_path):
         ds.PNGImageLoader(np.uint16, 1).load(p8)
     with pytest.raises(ValueError, match="CV_8UC3 or CV_8UC4"):
         ds.RGBLabelsLoader([[255, 0, 0]]).load(p8)
+
+
+def test_device_entry_points_check_ranges_before_launch():
+    """A byte range past the caller's buffer is an argument error, raised on
+    the host before any device work (no GPU needed to reach it)."""
+    import ctypes as C
+    from padenti_amd._lib import lib, ptr
+    off = np.array([0, 100], np.uint64)
+    n = np.array([100, 29], np.uint64)
+    ms = C.c_float()
+    fake = C.c_void_p(0x10000)   # never dereferenced: the check fails first
+    rc = lib().pd_crc32_images_dev(0, 2, fake, 128, ptr(off), ptr(n), fake, C.byref(ms))
+    assert rc == -1 and b"outside its buffer" in lib().pd_last_error()
+    smp = ds.UniformImageSampler(4)
+    w = np.array([10, 1], np.uint32)
+    h = np.array([10, 29], np.uint32)
+    with pytest.raises(PdError, match="image data range 1 lies outside"):
+        smp.sample_dev(2, fake, 128, off, n, fake, 1000, np.array([0, 112], np.uint64), w, h, fake, fake)
+    with pytest.raises(PdError, match="label image range 1 lies outside"):
+        smp.sample_dev(2, fake, 129, off, n, fake, 140, np.array([0, 112], np.uint64), w, h, fake, fake)
