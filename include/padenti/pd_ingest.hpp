@@ -92,61 +92,119 @@ __device__ __forceinline__ uint32_t crc_byte(const uint32_t (*T)[256], uint32_t 
 
 // One image's byte range [start, end) (absolute addresses) cut into 4 KiB
 // blocks from a0 = start & ~63; block m covers [a0 + 4096 m, a0 + 4096 (m+1)).
+// A wave hashes a chunk of up to CRC_CHUNK consecutive blocks.
+constexpr uint32_t CRC_CHUNK = 8;
+constexpr uint32_t CRC_GAP = 4096 - 64;  // bytes between a lane's segments in consecutive blocks
+
 struct CrcJob {
   uint64_t start, end, a0;
-  uint32_t first_block, n_blocks;
+  uint32_t first_chunk, n_blocks;
 };
 
-// Wave per 4 KiB block, lane per 64-byte segment (four 16-byte loads).
-// Lane CRCs are shifted to the end of the block's last whole segment and
-// xor-reduced; the last block continues the register over its tail bytes.
-// Blocks before the last one are shifted by P[n_blocks - 2 - m] =
-// x^(8 * 4096 (n_blocks - 2 - m)) into acc_a (the remaining shift, the
-// last block's length, is applied once per image by k_crc_final); the last
-// block goes to acc_b.  xor is order free, so the result is deterministic.
-__global__ __launch_bounds__(256) void k_crc32(const CrcJob *__restrict__ jobs, const uint32_t *__restrict__ blk_img,
-                                               uint32_t n_blocks, const uint32_t *__restrict__ P, uint32_t *acc_a,
-                                               uint32_t *acc_b)
+// Largest i < n with first[i] <= v (first ascending, first[0] = 0).
+__device__ __forceinline__ uint32_t find_owner(const uint32_t *__restrict__ first, uint32_t n, uint32_t v)
 {
-  __shared__ uint32_t T[4][256];
-  for (uint32_t i = threadIdx.x; i < 1024; i += 256) T[i >> 8][i & 255] = c_crc.t[i >> 8][i & 255];
+  uint32_t lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (first[mid] <= v) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ uint32_t crc_seg(const uint32_t (*T)[256], uint32_t crc, const uint4 (&v)[4])
+{
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    crc = crc_word(T, crc, v[q].x);
+    crc = crc_word(T, crc, v[q].y);
+    crc = crc_word(T, crc, v[q].z);
+    crc = crc_word(T, crc, v[q].w);
+  }
+  return crc;
+}
+
+// a * x^(8 CRC_GAP) mod P from G[k][v] = (v at byte k) * x^(8 CRC_GAP)
+__device__ __forceinline__ uint32_t mul_gap(const uint32_t (*G)[256], uint32_t a)
+{
+  return G[0][a & 0xFFu] ^ G[1][(a >> 8) & 0xFFu] ^ G[2][(a >> 16) & 0xFFu] ^ G[3][a >> 24];
+}
+
+// Wave per chunk, lane per 64-byte segment of every block (four 16-byte
+// loads, the next block's prefetched while the current one is hashed).  A
+// lane carries one register across the chunk's whole blocks: before each
+// new segment it is shifted over the 4032 bytes of the other lanes (table
+// multiply).  At the end of the whole blocks the lane registers are shifted
+// to the block end (x^(8 * 64 * j)), xor-reduced, and shifted by
+// P[n_blocks - 1 - e] = x^(8 * 4096 (n_blocks - 1 - e)) (e = the chunk's end
+// block) into acc_a; the image's last block (lanes whose segment ends in the
+// image, then the tail bytes) goes to acc_b.  k_crc_final applies the last
+// block's length.  xor is order free, so the result is deterministic.
+__global__ __launch_bounds__(256) void k_crc32(const CrcJob *__restrict__ jobs, const uint32_t *__restrict__ chunk0,
+                                               uint32_t n_images, uint32_t n_chunks, const uint32_t *__restrict__ P,
+                                               const uint32_t *__restrict__ gap, uint32_t *acc_a, uint32_t *acc_b)
+{
+  __shared__ uint32_t T[4][256], G[4][256];
+  for (uint32_t i = threadIdx.x; i < 1024; i += 256) {
+    T[i >> 8][i & 255] = c_crc.t[i >> 8][i & 255];
+    G[i >> 8][i & 255] = gap[i];
+  }
   __syncthreads();
   const uint32_t lane = threadIdx.x & 63u;
-  for (uint32_t b = blockIdx.x * 4u + (threadIdx.x >> 6); b < n_blocks; b += gridDim.x * 4u) {
-    const uint32_t im = blk_img[b];
+  for (uint32_t ch = blockIdx.x * 4u + (threadIdx.x >> 6); ch < n_chunks; ch += gridDim.x * 4u) {
+    const uint32_t im = find_owner(chunk0, n_images, ch);
     const CrcJob J = jobs[im];
-    const uint32_t m = b - J.first_block;
-    const bool last = (m + 1u == J.n_blocks);
-    const uint64_t blo = J.a0 + (uint64_t)m * 4096u;
-    const uint32_t L = last ? (uint32_t)((J.end - blo) >> 6) : 64u;  // lanes whose segment ends in the image
-    uint32_t crc = 0;
-    if (lane < L) {
-      const uint64_t lo = blo + 64ull * lane;
-      if (lo >= J.start) {
-        const uint4 *p = reinterpret_cast<const uint4 *>(lo);
-        const uint4 v0 = p[0], v1 = p[1], v2 = p[2], v3 = p[3];
-        crc = crc_word(T, crc, v0.x); crc = crc_word(T, crc, v0.y);
-        crc = crc_word(T, crc, v0.z); crc = crc_word(T, crc, v0.w);
-        crc = crc_word(T, crc, v1.x); crc = crc_word(T, crc, v1.y);
-        crc = crc_word(T, crc, v1.z); crc = crc_word(T, crc, v1.w);
-        crc = crc_word(T, crc, v2.x); crc = crc_word(T, crc, v2.y);
-        crc = crc_word(T, crc, v2.z); crc = crc_word(T, crc, v2.w);
-        crc = crc_word(T, crc, v3.x); crc = crc_word(T, crc, v3.y);
-        crc = crc_word(T, crc, v3.z); crc = crc_word(T, crc, v3.w);
-      } else {  // the image's first segment starts inside it
-        for (uint64_t a = J.start; a < lo + 64u; a++) crc = crc_byte(T, crc, *reinterpret_cast<const uint8_t *>(a));
+    const uint32_t m0 = (ch - J.first_chunk) * CRC_CHUNK;
+    const uint32_t m1 = (m0 + CRC_CHUNK < J.n_blocks) ? m0 + CRC_CHUNK : J.n_blocks;
+    const bool has_last = (m1 == J.n_blocks);
+    const uint32_t mf = has_last ? J.n_blocks - 1u : m1;  // whole blocks [m0, mf)
+    const bool head = (J.start > J.a0) && lane == 0;        // lane 0 of block 0 starts inside its segment
+    if (m0 < mf) {
+      uint4 v[4] = {make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0)};
+      if (!(m0 == 0 && head)) {
+        const uint4 *p = reinterpret_cast<const uint4 *>(J.a0 + (uint64_t)m0 * 4096u + 64ull * lane);
+        v[0] = p[0]; v[1] = p[1]; v[2] = p[2]; v[3] = p[3];
       }
-      crc = gf2_mul(crc, c_crc.w64[L - 1u - lane]);
+      uint32_t reg = 0;
+      for (uint32_t m = m0; m < mf; m++) {
+        const uint4 cur[4] = {v[0], v[1], v[2], v[3]};
+        if (m + 1u < mf) {
+          const uint4 *p = reinterpret_cast<const uint4 *>(J.a0 + (uint64_t)(m + 1u) * 4096u + 64ull * lane);
+          v[0] = p[0]; v[1] = p[1]; v[2] = p[2]; v[3] = p[3];
+        }
+        if (m > m0) reg = mul_gap(G, reg);
+        if (m == 0 && head) {
+          for (uint64_t a = J.start; a < J.a0 + 64u; a++) reg = crc_byte(T, reg, *reinterpret_cast<const uint8_t *>(a));
+        } else {
+          reg = crc_seg(T, reg, cur);
+        }
+      }
+      reg = gf2_mul(reg, c_crc.w64[63u - lane]);
+      for (int o = 32; o >= 1; o >>= 1) reg ^= __shfl_xor(reg, o);
+      if (lane == 0) atomicXor(&acc_a[im], gf2_mul(reg, P[J.n_blocks - 1u - mf]));
     }
-    for (int o = 32; o >= 1; o >>= 1) crc ^= __shfl_xor(crc, o);
-    if (last) {
+    if (has_last) {
+      const uint32_t m = J.n_blocks - 1u;
+      const uint64_t blo = J.a0 + (uint64_t)m * 4096u;
+      const uint32_t L = (uint32_t)((J.end - blo) >> 6);  // lanes whose segment ends in the image
+      uint32_t crc = 0;
+      if (lane < L) {
+        const uint64_t lo = blo + 64ull * lane;
+        if (lo >= J.start) {
+          const uint4 *p = reinterpret_cast<const uint4 *>(lo);
+          const uint4 w[4] = {p[0], p[1], p[2], p[3]};
+          crc = crc_seg(T, crc, w);
+        } else {  // the image's first segment starts inside it
+          for (uint64_t a = J.start; a < lo + 64u; a++) crc = crc_byte(T, crc, *reinterpret_cast<const uint8_t *>(a));
+        }
+        crc = gf2_mul(crc, c_crc.w64[L - 1u - lane]);
+      }
+      for (int o = 32; o >= 1; o >>= 1) crc ^= __shfl_xor(crc, o);
       const uint64_t t0 = blo + 64ull * L;
       for (uint64_t a = (t0 > J.start ? t0 : J.start); a < J.end; a++)
         crc = crc_byte(T, crc, *reinterpret_cast<const uint8_t *>(a));
       if (lane == 0) atomicXor(&acc_b[im], crc);
-    } else {
-      crc = gf2_mul(crc, P[J.n_blocks - 2u - m]);
-      if (lane == 0) atomicXor(&acc_a[im], crc);
     }
   }
 }
@@ -230,19 +288,25 @@ struct SampleJob {
   uint32_t pad;
 };
 
-// bits[word0 + w] = labelled-pixel mask of pixels [32 w, 32 w + 32); cnt = its popcount
+// bits[word0 + w] = labelled-pixel mask of pixels [32 w, 32 w + 32); cnt = its
+// popcount.  A block per tile of 256 words of one image; tile0[i] = first
+// tile of image i (binary search, n_images entries).
 __global__ __launch_bounds__(256) void k_label_bits(const uint8_t *__restrict__ labels,
                                                     const SampleJob *__restrict__ jobs,
-                                                    const uint32_t *__restrict__ word_img, uint32_t n_words,
+                                                    const uint32_t *__restrict__ tile0, uint32_t n_images,
                                                     uint32_t *bits, uint32_t *cnt)
 {
-  const uint32_t g = blockIdx.x * 256u + threadIdx.x;
-  if (g >= n_words) return;
-  const SampleJob J = jobs[word_img[g]];
-  const uint32_t w = g - J.word0;
+  __shared__ uint32_t s_im;
+  if (threadIdx.x == 0) s_im = find_owner(tile0, n_images, blockIdx.x);
+  __syncthreads();
+  const uint32_t im = s_im;
+  const SampleJob J = jobs[im];
+  const uint32_t w = (blockIdx.x - tile0[im]) * 256u + threadIdx.x;
+  if (32ull * w >= J.n_px) return;
+  const uint32_t g = J.word0 + w;
   const uint8_t *lab = labels + J.lab_off + 32ull * w;
   uint32_t m = 0;
-  if (32u * w + 32u <= J.n_px) {
+  if (32ull * w + 32u <= J.n_px) {
     const uint4 a = reinterpret_cast<const uint4 *>(lab)[0], b = reinterpret_cast<const uint4 *>(lab)[1];
     const uint32_t v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
 #pragma unroll
@@ -347,100 +411,148 @@ __global__ __launch_bounds__(256) void k_sample_uniform(const SampleJob *__restr
 }
 
 // ---- class-weighted reservoir --------------------------------------------
-// One block of 256 threads per image, 256 pixels per step in scan order.
-// bucket: zero-initialised slots [image][sum B] (slot of class c at boff[c]),
-// filled with atomicMax (the last writer in scan order has the largest pixel
-// index).  Labels above n_classes set *bad.
+// Ordering inside one wave: LDS operations of a wave execute in issue order,
+// so a fence + wave barrier is all the synchronisation lanes of one wave need.
+__device__ __forceinline__ void wave_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One twist of a state by one wave (no tempering: outputs are tempered when
+// drawn).  Same three dependent ranges as mt_twist.
+__device__ inline void mt_twist_wave(uint32_t *mt, uint32_t lane)
+{
+  uint32_t a1[4], a2[4], a3[3];
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const uint32_t i = lane + 64u * r;
+    a1[r] = a2[r] = 0;
+    if (i < 227) {
+      a1[r] = mt[i + MT_M] ^ mt_mix(mt[i], mt[i + 1]);
+      a2[r] = mt_mix(mt[227 + i], mt[228 + i]);
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 3; r++) {
+    const uint32_t i = lane + 64u * r;
+    a3[r] = (i < 169) ? mt_mix(mt[454 + i], mt[455 + i]) : 0u;
+  }
+  const uint32_t last_old = mt[623];
+  wave_sync();
+#pragma unroll
+  for (int r = 0; r < 4; r++)
+    if (lane + 64u * r < 227) mt[lane + 64u * r] = a1[r];
+  wave_sync();
+#pragma unroll
+  for (int r = 0; r < 4; r++)
+    if (lane + 64u * r < 227) mt[227 + lane + 64u * r] = mt[lane + 64u * r] ^ a2[r];
+  wave_sync();
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+    if (lane + 64u * r < 169) mt[454 + lane + 64u * r] = mt[227 + lane + 64u * r] ^ a3[r];
+  if (lane == 0) mt[623] = mt[396] ^ mt_mix(last_old, mt[0]);
+  wave_sync();
+}
+
+// One wave per image (four images per block), 64 pixels per step in scan
+// order (lane l: pixel t0 + 64 s + l); the label bytes of the next 16 steps
+// are prefetched from global memory.  A pixel's class ordinal k is the class
+// count so far (LDS) plus its rank among the wave's lanes of that class (one
+// ballot per distinct class, the class read with readlane); it draws iff
+// k >= B_c, and its draw index is the wave prefix of those flags, so the MT
+// stream is consumed in the serial loop's order.  bucket: zero-initialised
+// slots [image][sum B] (class c at boff[c]) filled with atomicMax (the last
+// writer in scan order has the largest pixel index).  Labels above
+// n_classes set *bad.
+//
+// Bound by VALU issue: about 85 VALU + 43 SALU instructions per 64-pixel step
+// (class ranks, the draw's FP64 product and roundf, tempering, a twist per
+// 624 draws).  Keeping the slots in LDS instead of global atomics cut the
+// occupancy and ran slower.
 __global__ __launch_bounds__(256) void k_sample_weighted(const uint8_t *__restrict__ labels,
-                                                         const SampleJob *__restrict__ jobs,
+                                                         const SampleJob *__restrict__ jobs, uint32_t n_images,
                                                          const uint32_t *__restrict__ crc, uint32_t seed,
                                                          uint32_t n_classes, const uint32_t *__restrict__ B,
                                                          const uint32_t *__restrict__ boff, uint32_t *bucket,
                                                          uint32_t *class_count, uint32_t *bad)
 {
-  __shared__ uint32_t mt[MT_N], y[2 * MT_N];
-  __shared__ uint32_t cnt[256], cntw[4][256], sB[256], dw[4];
-  __shared__ uint4 tile[256];
-  const uint32_t im = blockIdx.x, tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+  __shared__ uint32_t sB[256], sO[256];
+  __shared__ uint32_t mt_all[4][MT_N], cnt_all[4][256];
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  sB[threadIdx.x] = threadIdx.x < n_classes ? B[threadIdx.x] : 0u;
+  sO[threadIdx.x] = threadIdx.x < n_classes ? boff[threadIdx.x] : 0u;
+  __syncthreads();  // the only block-wide barrier: from here on the waves are independent
+  const uint32_t im = blockIdx.x * 4u + wave;
+  if (im >= n_images) return;
+  uint32_t *mt = mt_all[wave], *cnt = cnt_all[wave];
+  for (uint32_t c = lane; c < 256u; c += 64u) cnt[c] = 0;
   const SampleJob J = jobs[im];
   const uint8_t *lab = labels + J.lab_off;
   uint32_t *bk = bucket + J.out_base;
-  cnt[tid] = 0;
-  cntw[0][tid] = cntw[1][tid] = cntw[2][tid] = cntw[3][tid] = 0;
-  sB[tid] = tid < n_classes ? B[tid] : 0u;
-  if (tid == 0) mt_seed(mt, crc[im] ^ seed);
-  __syncthreads();
-  uint32_t D = 0, gen_end = 0;  // draws consumed; outputs generated so far
+  if (lane == 0) mt_seed(mt, crc[im] ^ seed);
+  wave_sync();
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
-  for (uint32_t t0 = 0; t0 < J.n_px; t0 += 4096u) {
-    {  // stage 4096 label bytes
-      const uint32_t b0 = t0 + 16u * tid;
-      uint4 v = make_uint4(0, 0, 0, 0);
-      if (b0 + 16u <= J.n_px) {
-        v = reinterpret_cast<const uint4 *>(lab + b0)[0];
-      } else if (b0 < J.n_px) {
-        uint32_t w[4] = {0, 0, 0, 0};
-        for (uint32_t j = 0; b0 + j < J.n_px; j++) w[j >> 2] |= (uint32_t)lab[b0 + j] << (8 * (j & 3));
-        v = make_uint4(w[0], w[1], w[2], w[3]);
-      }
-      tile[tid] = v;
+  uint32_t D = 0, gen = 0;  // draws consumed; outputs generated ([gen - 624, gen) are in mt)
+  uint32_t nx[16];
+#pragma unroll
+  for (int s = 0; s < 16; s++) nx[s] = (64u * s + lane < J.n_px) ? lab[64u * s + lane] : 0u;
+  for (uint32_t t0 = 0; t0 < J.n_px; t0 += 1024u) {
+    uint32_t cur[16];
+#pragma unroll
+    for (int s = 0; s < 16; s++) {
+      cur[s] = nx[s];
+      const uint32_t q = t0 + 1024u + 64u * s + lane;
+      nx[s] = (q < J.n_px) ? lab[q] : 0u;  // prefetch the next tile
     }
-    __syncthreads();
-    const uint8_t *tb = reinterpret_cast<const uint8_t *>(tile);
-    for (uint32_t s = 0; s < 16u && t0 + 256u * s < J.n_px; s++) {
-      const uint32_t p = t0 + 256u * s + tid;
-      uint32_t l = (p < J.n_px) ? tb[256u * s + tid] : 0u;
+#pragma unroll
+    for (int s = 0; s < 16; s++) {
+      if (t0 + 64u * s >= J.n_px) break;
+      const uint32_t p = t0 + 64u * s + lane;
+      uint32_t l = cur[s];
       if (l > n_classes) { atomicOr(bad, 1u); l = 0; }
       const bool valid = l != 0u;
       const uint32_t c = l - 1u;
-      // class ordinal inside the wave: one ballot per distinct class
+      uint32_t k = 0, Bc = 0, Oc = 0;
       uint64_t pending = __ballot(valid);
-      uint32_t rank = 0;
-      bool leader = false;
       while (pending) {
         const uint32_t lead = (uint32_t)__builtin_ctzll(pending);
-        const uint32_t cl = __shfl(c, lead);
+        const uint32_t cl = __builtin_amdgcn_readlane(c, lead);
         const uint64_t mm = __ballot(valid && c == cl);
-        if (valid && c == cl) rank = (uint32_t)__popcll(mm & lt);
-        if (lane == lead) { cntw[wave][cl] = (uint32_t)__popcll(mm); leader = true; }
+        const uint32_t before = cnt[cl], bcl = sB[cl], ocl = sO[cl];
+        if (valid && c == cl) {
+          k = before + (uint32_t)__popcll(mm & lt);
+          Bc = bcl;
+          Oc = ocl;
+        }
+        if (lane == lead) cnt[cl] = before + (uint32_t)__popcll(mm);
         pending &= ~mm;
       }
-      __syncthreads();
-      uint32_t k = 0;
-      if (valid) {
-        k = cnt[c] + rank;
-        for (uint32_t w = 0; w < wave; w++) k += cntw[w][c];
-      }
-      const bool draw = valid && k >= sB[c];
+      const bool draw = valid && k >= Bc;
       const uint64_t dm = __ballot(draw);
-      if (lane == 0) dw[wave] = (uint32_t)__popcll(dm);
-      __syncthreads();
-      uint32_t d = D + (uint32_t)__popcll(dm & lt), total = 0;
-      for (uint32_t w = 0; w < 4; w++) {
-        if (w < wave) d += dw[w];
-        total += dw[w];
-      }
-      if (D + total > gen_end) {  // block-uniform: one twist covers the <= 256 draws of a step
-        mt_twist(mt, y + (gen_end % (2u * MT_N)), tid);
-        gen_end += MT_N;
+      const uint32_t d = D + (uint32_t)__popcll(dm & lt);
+      D += (uint32_t)__popcll(dm);
+      uint32_t x = 0;
+      if (D > gen) {  // wave-uniform: this step crosses into the next twist
+        if (draw && d < gen) x = mt[d + MT_N - gen];
+        mt_twist_wave(mt, lane);
+        gen += MT_N;
+        if (draw && d + MT_N >= gen) x = mt[d + MT_N - gen];
+      } else if (draw) {
+        x = mt[d + MT_N - gen];
       }
       if (draw) {
-        const double u = (double)y[d % (2u * MT_N)] * 2.3283064365386962890625e-10;  // x / 2^32
+        const double u = (double)mt_temper(x) * 2.3283064365386962890625e-10;  // x / 2^32
         const uint32_t j = (uint32_t)roundf((float)(u * (double)k));
-        if (j < sB[c]) atomicMax(&bk[boff[c] + j], p);
+        if (j < Bc) atomicMax(&bk[Oc + j], p);
       } else if (valid) {
-        atomicMax(&bk[boff[c] + k], p);
-      }
-      D += total;
-      __syncthreads();
-      if (leader) {
-        atomicAdd(&cnt[c], cntw[wave][c]);
-        cntw[wave][c] = 0;
+        atomicMax(&bk[Oc + k], p);
       }
     }
-    __syncthreads();
   }
-  if (tid < n_classes) class_count[(uint64_t)im * n_classes + tid] = cnt[tid];
+  wave_sync();
+  for (uint32_t c = lane; c < n_classes; c += 64u) class_count[(uint64_t)im * n_classes + c] = cnt[c];
 }
 
 // filled slots of every class -> contiguous unsorted samples; segment end

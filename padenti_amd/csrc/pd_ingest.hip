@@ -27,12 +27,13 @@ namespace {
 struct IngestDev {
   std::recursive_mutex mu;
   hipStream_t stream = nullptr;
-  DevBuf jobs, blk_img, P, acc, crc, sjobs, word_img, bits, cnt, pre, scan_tmp, unsorted, bucket, class_count, B,
+  DevBuf jobs, blk0, P, gap, acc, crc, sjobs, tile0, bits, cnt, pre, scan_tmp, unsorted, bucket, class_count, B,
       boff, seg_begin, seg_end, sort_tmp, bad, data, labels, samples, n_out, crc_out, keys, cls, rgb, lab_out;
   std::vector<uint32_t> hP;  // x^(8 * 4096 j) mod P
+  std::vector<uint32_t> hgap;  // [4][256]: byte k of a register times x^(8 * CRC_GAP)
   std::vector<CrcJob> h_jobs;
   std::vector<SampleJob> h_sjobs;
-  std::vector<uint32_t> h_blk, h_word_img, h_seg_begin, h_B, h_boff, h_keys;
+  std::vector<uint32_t> h_blk0, h_tile0, h_seg_begin, h_B, h_boff, h_keys;
   std::vector<uint8_t> h_cls, h_stage;
 };
 
@@ -106,9 +107,10 @@ void launch_crc(IngestDev &X, uint32_t n, const uint8_t *base, const uint64_t *o
                 uint32_t *d_crc)
 {
   std::vector<CrcJob> &jobs = X.h_jobs;
-  std::vector<uint32_t> &blk = X.h_blk;
+  std::vector<uint32_t> &chunk0 = X.h_blk0;
   jobs.assign(n, CrcJob{});
-  blk.clear();
+  chunk0.assign(n, 0u);
+  uint64_t total = 0;  // chunks
   uint32_t max_nb = 0;
   for (uint32_t i = 0; i < n; i++) {
     CrcJob &J = jobs[i];
@@ -116,23 +118,30 @@ void launch_crc(IngestDev &X, uint32_t n, const uint8_t *base, const uint64_t *o
     J.end = J.start + bytes[i];
     J.a0 = J.start & ~63ull;
     const uint64_t nb = bytes[i] ? (J.end - J.a0 + 4095u) / 4096u : 0u;
-    require(blk.size() + nb < (1ull << 32), "too many bytes in one CRC batch");
-    J.first_block = (uint32_t)blk.size();
+    require(nb < (1ull << 32) && total + nb < (1ull << 32), "too many bytes in one CRC batch");
+    J.first_chunk = chunk0[i] = (uint32_t)total;
     J.n_blocks = (uint32_t)nb;
-    blk.insert(blk.end(), (size_t)nb, i);
+    total += (nb + CRC_CHUNK - 1) / CRC_CHUNK;
     max_nb = std::max<uint32_t>(max_nb, (uint32_t)nb);
   }
   if (X.hP.empty()) X.hP.push_back(CRC_ONE);
   while (X.hP.size() < max_nb) X.hP.push_back(gf2_mul(X.hP.back(), HT.x2k[15]));  // * x^(8 * 4096)
+  if (X.hgap.empty()) {
+    const uint32_t C = crc_x8n(HT, CRC_GAP);
+    X.hgap.resize(1024);
+    for (uint32_t k = 0; k < 4; k++)
+      for (uint32_t v = 0; v < 256; v++) X.hgap[k * 256 + v] = gf2_mul(v << (8 * k), C);
+  }
   upload(X.jobs, jobs, X.stream);
-  upload(X.blk_img, blk, X.stream);
+  upload(X.blk0, chunk0, X.stream);
   upload(X.P, X.hP, X.stream);
+  upload(X.gap, X.hgap, X.stream);
   X.acc.ensure(2ull * std::max<uint32_t>(n, 1) * 4u);
   uint32_t *acc = X.acc.as<uint32_t>();
   hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(2ull * n, 256)), dim3(256), 0, X.stream, acc, 2ull * n, 0u);
-  if (!blk.empty())
-    hipLaunchKernelGGL(k_crc32, dim3(grid_for(blk.size(), 4, 1u << 20)), dim3(256), 0, X.stream,
-                       X.jobs.as<CrcJob>(), X.blk_img.as<uint32_t>(), (uint32_t)blk.size(), X.P.as<uint32_t>(), acc,
+  if (total)
+    hipLaunchKernelGGL(k_crc32, dim3(grid_for(total, 4, 1u << 20)), dim3(256), 0, X.stream, X.jobs.as<CrcJob>(),
+                       X.blk0.as<uint32_t>(), n, (uint32_t)total, X.P.as<uint32_t>(), X.gap.as<uint32_t>(), acc,
                        acc + n);
   hipLaunchKernelGGL(k_crc_final, dim3(grid_for(n, 64)), dim3(64), 0, X.stream, X.jobs.as<CrcJob>(), n, acc,
                      acc + n, d_crc);
@@ -169,14 +178,15 @@ void sample_dev(int device, const pd_sampler &s, uint32_t n, const void *d_data,
   const hipStream_t st = X.stream;
 
   std::vector<SampleJob> &jobs = X.h_sjobs;
-  std::vector<uint32_t> &word_img = X.h_word_img, &seg_begin = X.h_seg_begin;
+  std::vector<uint32_t> &tile0 = X.h_tile0, &seg_begin = X.h_seg_begin;
   jobs.assign(n, SampleJob{});
-  word_img.clear();
+  tile0.assign(n, 0u);
   seg_begin.assign(n, 0u);
+  uint64_t n_words = 0, n_tiles = 0;
   uint64_t label_bytes = 0, crc_bytes = 0, max_px = 0;
   for (uint32_t i = 0; i < n; i++) {
     const uint64_t px = (uint64_t)widths[i] * heights[i];
-    require(px < (1ull << 32) - 32, "an image must have fewer than 2^32 - 32 pixels");
+    require(px < (1ull << 31), "an image must have fewer than 2^31 pixels");
     require(((uintptr_t)d_labels + lab_off[i]) % 16 == 0, "label images must start at 16-byte aligned addresses");
     SampleJob &J = jobs[i];
     J.lab_off = lab_off[i];
@@ -187,16 +197,13 @@ void sample_dev(int device, const pd_sampler &s, uint32_t n, const void *d_data,
     label_bytes += px;
     crc_bytes += data_bytes[i];
     max_px = std::max(max_px, px);
-    if (s.kind == PD_SAMPLER_UNIFORM) {
-      const uint64_t nw = (px + 31) / 32;
-      require(word_img.size() + nw < (1ull << 32) - 1, "too many pixels in one batch");
-      J.word0 = (uint32_t)word_img.size();
-      word_img.insert(word_img.end(), (size_t)nw, i);
-    } else {
-      J.word0 = 0;
-    }
+    const uint64_t nw = (px + 31) / 32;
+    J.word0 = (uint32_t)n_words;
+    tile0[i] = (uint32_t)n_tiles;
+    n_words += nw;
+    n_tiles += (nw + 255) / 256;
   }
-  if (s.kind == PD_SAMPLER_UNIFORM) require(label_bytes < (1ull << 32), "more than 2^32 pixels in one batch");
+  require(n_words < (1ull << 31) && n_tiles < (1ull << 31), "too many pixels in one batch");
 
   Timer tm(st, 5);
   tm.mark(0);
@@ -210,16 +217,16 @@ void sample_dev(int device, const pd_sampler &s, uint32_t n, const void *d_data,
   X.unsorted.ensure((size_t)n * cap * 4u + 4u);
   uint32_t *uns = X.unsorted.as<uint32_t>();
   if (s.kind == PD_SAMPLER_UNIFORM) {
-    const uint32_t nw = (uint32_t)word_img.size();
-    upload(X.word_img, word_img, st);
+    const uint32_t nw = (uint32_t)n_words;
+    upload(X.tile0, tile0, st);
     X.bits.ensure((size_t)nw * 4u + 4u);
     X.cnt.ensure(((size_t)nw + 1) * 4u);
     X.pre.ensure(((size_t)nw + 1) * 4u);
     uint32_t *cnt = X.cnt.as<uint32_t>();
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, st, cnt + nw, 1ull, 0u);
-    if (nw)
-      hipLaunchKernelGGL(k_label_bits, dim3(grid_for(nw, 256, 1u << 30)), dim3(256), 0, st, d_labels,
-                         X.sjobs.as<SampleJob>(), X.word_img.as<uint32_t>(), nw, X.bits.as<uint32_t>(), cnt);
+    if (n_tiles)
+      hipLaunchKernelGGL(k_label_bits, dim3((uint32_t)n_tiles), dim3(256), 0, st, d_labels, X.sjobs.as<SampleJob>(),
+                         X.tile0.as<uint32_t>(), n, X.bits.as<uint32_t>(), cnt);
     size_t tb = 0;
     HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, cnt, X.pre.as<uint32_t>(), (int)nw + 1, st));
     X.scan_tmp.ensure(tb);
@@ -243,8 +250,8 @@ void sample_dev(int device, const pd_sampler &s, uint32_t n, const void *d_data,
                        X.bucket.as<uint32_t>(), (uint64_t)n * cap, 0u);
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, st, X.bad.as<uint32_t>(), 1ull, 0u);
     tm.mark(2);
-    hipLaunchKernelGGL(k_sample_weighted, dim3(n), dim3(256), 0, st, d_labels, X.sjobs.as<SampleJob>(), crc,
-                       s.seed, s.n_classes, X.B.as<uint32_t>(), X.boff.as<uint32_t>(), X.bucket.as<uint32_t>(),
+    hipLaunchKernelGGL(k_sample_weighted, dim3((n + 3) / 4), dim3(256), 0, st, d_labels, X.sjobs.as<SampleJob>(), n,
+                       crc, s.seed, s.n_classes, X.B.as<uint32_t>(), X.boff.as<uint32_t>(), X.bucket.as<uint32_t>(),
                        X.class_count.as<uint32_t>(), X.bad.as<uint32_t>());
     hipLaunchKernelGGL(k_compact_buckets, dim3(n), dim3(256), 0, st, X.sjobs.as<SampleJob>(), s.n_classes,
                        X.B.as<uint32_t>(), X.boff.as<uint32_t>(), X.class_count.as<uint32_t>(),

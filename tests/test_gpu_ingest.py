@@ -101,7 +101,8 @@ def test_uniform_sampler_matches_oracle(gpu, n_samples, seed):
 
 
 @pytest.mark.parametrize("n_classes,weights,n_samples", [
-    (8, None, 2000), (8, "random", 300), (8, "zeros", 64), (1, [1.0], 5), (8, "big", 50), (255, None, 3000)])
+    (8, None, 2000), (8, "random", 300), (8, "zeros", 64), (1, [1.0], 5), (8, "big", 50), (255, None, 3000),
+    (8, "big", 2000)])   # the last: 48000 slots per image
 def test_weighted_sampler_matches_oracle(gpu, n_classes, weights, n_samples):
     datas, labs = _batch(7 + n_classes + n_samples)
     rng = np.random.default_rng(n_samples)
