@@ -29,6 +29,15 @@
 
 #pragma clang fp contract(off)
 
+// A translation unit may define PD_KERNEL_SET_LIGHT to 1 before including
+// this header (feature plugins that instantiate many hooks): the histogram
+// kernel is then compiled for two threshold buckets (16 and 64; a larger
+// bucket pads with thresholds no finite response exceeds) and one block size
+// (8 waves, launch tuning is ignored), and prediction for two class buckets.
+#ifndef PD_KERNEL_SET_LIGHT
+#define PD_KERNEL_SET_LIGHT 0
+#endif
+
 namespace pd {
 namespace engine {
 
@@ -502,9 +511,13 @@ struct Trainer final : TrainerBase {
   void launch_hist_t(uint32_t n_items, uint32_t item_base, const uint4 *act, const Bounds<F> &bnd, uint32_t tree,
                      uint32_t nF, uint32_t nT)
   {
+#if PD_KERNEL_SET_LIGHT
+    launch_hist_w<NTP, 8>(n_items, item_base, act, bnd, tree, nF, nT);
+#else
     if (tune.waves == 4) launch_hist_w<NTP, 4>(n_items, item_base, act, bnd, tree, nF, nT);
     else if (tune.waves == 16) launch_hist_w<NTP, 16>(n_items, item_base, act, bnd, tree, nF, nT);
     else launch_hist_w<NTP, 8>(n_items, item_base, act, bnd, tree, nF, nT);
+#endif
   }
 
   template <int NTP, int NW>
@@ -528,6 +541,11 @@ struct Trainer final : TrainerBase {
                    uint32_t nF, uint32_t nT)
   {
     if (!n_items) return;
+#if PD_KERNEL_SET_LIGHT
+    if (nT <= 16) launch_hist_t<16>(n_items, item_base, act, bnd, tree, nF, nT);
+    else launch_hist_t<64>(n_items, item_base, act, bnd, tree, nF, nT);
+    return;
+#endif
     switch (n_thr_bucket(nT)) {
       case 1: launch_hist_t<1>(n_items, item_base, act, bnd, tree, nF, nT); break;
       case 8: launch_hist_t<8>(n_items, item_base, act, bnd, tree, nF, nT); break;
@@ -1323,9 +1341,15 @@ struct Forest final : ForestBase {
   void launch_any(uint32_t blocks, const I *pix, const FrameDesc *frames, uint32_t n_frames, const uint8_t *masks,
                   uint32_t tree0, uint32_t ntr, uint32_t steps, float *out_post, int32_t *out_leaf)
   {
+#if PD_KERNEL_SET_LIGHT
+    if (nC <= 16) launch<16>(blocks, pix, frames, n_frames, masks, tree0, ntr, steps, out_post, out_leaf);
+#else
     if (nC <= 8) launch<8>(blocks, pix, frames, n_frames, masks, tree0, ntr, steps, out_post, out_leaf);
     else if (nC <= 16) launch<16>(blocks, pix, frames, n_frames, masks, tree0, ntr, steps, out_post, out_leaf);
+#endif
+#if !PD_KERNEL_SET_LIGHT
     else if (nC <= 32) launch<32>(blocks, pix, frames, n_frames, masks, tree0, ntr, steps, out_post, out_leaf);
+#endif
     else launch<64>(blocks, pix, frames, n_frames, masks, tree0, ntr, steps, out_post, out_leaf);
   }
 

@@ -20,6 +20,8 @@
 // (hist_update.cl:154, predict.cl:75).
 #pragma once
 #include <math.h>
+#include <limits>
+#include <type_traits>
 #include "pd_math.hpp"
 
 #pragma clang fp contract(off)
@@ -268,12 +270,13 @@ PD_HD P unpack_prep(uint32_t z, uint32_t w)
 }
 
 // Threshold padding value: compares "left" against every response, so a
-// padded slot never counts as "right" for a finite response.
+// padded slot never counts as "right" for a finite response (+inf for the
+// floating types, the largest value for the integer ones).
 template <typename T>
-PD_HD T pad_threshold();
-template <>
-PD_HD float pad_threshold<float>() { return __builtin_inff(); }
-template <>
-PD_HD int16_t pad_threshold<int16_t>() { return (int16_t)32767; }
+PD_HD T pad_threshold()
+{
+  if constexpr (std::is_floating_point<T>::value) return (T)__builtin_inf();
+  else return std::numeric_limits<T>::max();
+}
 
 }  // namespace pd

@@ -88,8 +88,24 @@ enum {
   OR_FEAT_DEMO_S16 = 3     /* uint16 mm, FeatType short, FD 2 (test/feature.cl) */
 };
 
+/* Generic kinds: the ChannelDiff hook (padenti_amd/examples/channel_diff.hpp)
+ * for any image element type, channel count and FeatType:
+ *   kind = OR_GENERIC | nch << 8 | img_tag << 4 | feat_tag
+ * tags 0 int8, 1 uint8, 2 int16, 3 uint16, 4 int32, 5 uint32, 6 int64,
+ * 7 uint64, 8 float, 9 double.  FeatDim 4: (ox, oy, c1, c2). */
+#define OR_GENERIC 0x10000
+static int is_generic(int kind) { return (kind & OR_GENERIC) != 0; }
+static int gen_feat(int kind) { return kind & 15; }
+static int gen_img(int kind) { return (kind >> 4) & 15; }
+static size_t tag_size(int tag)
+{
+  static const size_t sz[10] = {1, 1, 2, 2, 4, 4, 8, 8, 4, 8};
+  return sz[tag];
+}
+
 static int feat_dim_of(int kind)
 {
+  if (is_generic(kind)) return 4;
   switch (kind) {
     case OR_FEAT_DEPTH_F32: return 4;
     case OR_FEAT_DEPTH_U16: return 4;
@@ -99,6 +115,36 @@ static int feat_dim_of(int kind)
   return -1;
 }
 static int feat_is_short(int kind) { return kind == OR_FEAT_DEMO_S16; }
+static size_t feat_size(int kind)
+{
+  if (is_generic(kind)) return tag_size(gen_feat(kind));
+  return feat_is_short(kind) ? 2 : 4;
+}
+
+/* lo + (T)(((float)w)/2^32 * (hi - lo)) in FeatType T with C's conversion
+ * rules (the span is int for types narrower than int, the product float for
+ * every integer type and double for double); bounds arrive as double. */
+#define OR_SAMPLE_T(T)                                                        \
+  {                                                                         \
+    T lo_ = (T)lo, hi_ = (T)hi;                                             \
+    float u = (float)w * 0x1p-32f;                                          \
+    *(T *)out = (T)(lo_ + (T)(u * (hi_ - lo_)));                            \
+  }
+static void sample_tagged(int tag, uint32_t w, double lo, double hi, void *out)
+{
+  switch (tag) {
+    case 0: OR_SAMPLE_T(int8_t) break;
+    case 1: OR_SAMPLE_T(uint8_t) break;
+    case 2: OR_SAMPLE_T(int16_t) break;
+    case 3: OR_SAMPLE_T(uint16_t) break;
+    case 4: OR_SAMPLE_T(int32_t) break;
+    case 5: OR_SAMPLE_T(uint32_t) break;
+    case 6: OR_SAMPLE_T(int64_t) break;
+    case 7: OR_SAMPLE_T(uint64_t) break;
+    case 8: OR_SAMPLE_T(float) break;
+    case 9: OR_SAMPLE_T(double) break;
+  }
+}
 
 /* A2/A3: lo + (T)(((float)w)/0xFFFFFFFF * (hi - lo))
  * hist_update.cl:94-111 (params), :150-183 (thresholds);
@@ -129,7 +175,9 @@ void or_feature_params(int kind, uint32_t tree, uint32_t node, uint32_t f,
   uint32_t seed[4] = {tree, node, f, 0u}, st[4];
   for (int j = 0; j < fd; j++) {
     if ((j & 3) == 0) { or_md5rand(seed, st); memcpy(seed, st, sizeof(st)); }
-    if (feat_is_short(kind))
+    if (is_generic(kind))
+      sample_tagged(gen_feat(kind), seed[j & 3], lo[j], hi[j], (char *)out + j * feat_size(kind));
+    else if (feat_is_short(kind))
       ((int16_t *)out)[j] = or_sample_s16(seed[j & 3], (int16_t)lo[j], (int16_t)hi[j]);
     else
       ((float *)out)[j] = or_sample_f32(seed[j & 3], (float)lo[j], (float)hi[j]);
@@ -144,7 +192,9 @@ void or_thresholds(int kind, uint32_t tree, uint32_t node, uint32_t f, uint32_t 
   uint32_t seed[4] = {tree, node, f, 1u}, st[4];
   for (uint32_t t = 0; t < nT; t++) {
     if ((t & 3) == 0) { or_md5rand(seed, st); memcpy(seed, st, sizeof(st)); }
-    if (feat_is_short(kind))
+    if (is_generic(kind))
+      sample_tagged(gen_feat(kind), seed[t & 3], lo, hi, (char *)out + t * feat_size(kind));
+    else if (feat_is_short(kind))
       ((int16_t *)out)[t] = or_sample_s16(seed[t & 3], (int16_t)lo, (int16_t)hi);
     else
       ((float *)out)[t] = or_sample_f32(seed[t & 3], (float)lo, (float)hi);
@@ -251,10 +301,92 @@ static int16_t feat_demo_s16(const or_image *im, int x, int y, const int16_t *p)
   return response;
 }
 
-/* Evaluate hook `kind`; returns the response widened to double (exact for
- * float and short).  `p` holds FeatType params. */
-double or_feature(int kind, const or_image *im, int x, int y, const void *p)
+/* A pixel value of the generic image, carried without loss: integers as
+ * int64 (every element type here fits), floating values as double, so that
+ * the conversion to FeatType below is C's conversion from the element type. */
+typedef struct { int is_int; int64_t i; double d; } or_px;
+static or_px px_tagged(const or_image *im, int tag, int x, int y, int c)
 {
+  size_t k = ((size_t)y * im->w + x) * im->nch + c;
+  or_px v = {1, 0, 0.0};
+  switch (tag) {
+    case 0: v.i = ((const int8_t *)im->data)[k]; break;
+    case 1: v.i = ((const uint8_t *)im->data)[k]; break;
+    case 2: v.i = ((const int16_t *)im->data)[k]; break;
+    case 3: v.i = ((const uint16_t *)im->data)[k]; break;
+    case 4: v.i = ((const int32_t *)im->data)[k]; break;
+    case 5: v.i = ((const uint32_t *)im->data)[k]; break;
+    case 6: v.i = ((const int64_t *)im->data)[k]; break;
+    case 8: v.is_int = 0; v.d = ((const float *)im->data)[k]; break;
+    case 9: v.is_int = 0; v.d = ((const double *)im->data)[k]; break;
+  }
+  return v;
+}
+
+/* ChannelDiff: a = I(x + (int)ox, y + (int)oy, c1 mod nch) (0 outside),
+ * b = I(x, y, c2 mod nch), response (T)((T)a - (T)b). */
+#define OR_CHANNEL_DIFF_T(T)                                                                      \
+  {                                                                                             \
+    const T *q = (const T *)p;                                                                  \
+    int ox = x + (int)q[0], oy = y + (int)q[1];                                                  \
+    int c1 = (int)((uint32_t)q[2] % (uint32_t)im->nch), c2 = (int)((uint32_t)q[3] % (uint32_t)im->nch); \
+    T a = (T)0, b;                                                                              \
+    if (ox >= 0 && ox < im->w && oy >= 0 && oy < im->h) {                                       \
+      or_px v = px_tagged(im, itag, ox, oy, c1);                                                \
+      a = v.is_int ? (T)v.i : (T)v.d;                                                           \
+    }                                                                                           \
+    or_px u = px_tagged(im, itag, x, y, c2);                                                    \
+    b = u.is_int ? (T)u.i : (T)u.d;                                                             \
+    T r = (T)(a - b);                                                                           \
+    if (out) memcpy(out, &r, sizeof(T));                                                        \
+    return (long double)r;                                                                      \
+  }
+static long double feat_channel_diff(int kind, const or_image *im, int x, int y, const void *p, void *out)
+{
+  int itag = gen_img(kind);
+  switch (gen_feat(kind)) {
+    case 0: OR_CHANNEL_DIFF_T(int8_t)
+    case 1: OR_CHANNEL_DIFF_T(uint8_t)
+    case 2: OR_CHANNEL_DIFF_T(int16_t)
+    case 3: OR_CHANNEL_DIFF_T(uint16_t)
+    case 4: OR_CHANNEL_DIFF_T(int32_t)
+    case 5: OR_CHANNEL_DIFF_T(uint32_t)
+    case 6: OR_CHANNEL_DIFF_T(int64_t)
+    case 7: OR_CHANNEL_DIFF_T(uint64_t)
+    case 8: OR_CHANNEL_DIFF_T(float)
+    case 9: OR_CHANNEL_DIFF_T(double)
+  }
+  return 0.0L;
+}
+
+/* a FeatType value widened to long double (exact for every FeatType: the
+ * x86-64 long double has a 64-bit significand) */
+static long double feat_ld(int kind, const void *base, uint32_t i)
+{
+  if (!is_generic(kind)) {
+    if (feat_is_short(kind)) return ((const int16_t *)base)[i];
+    return ((const float *)base)[i];
+  }
+  switch (gen_feat(kind)) {
+    case 0: return ((const int8_t *)base)[i];
+    case 1: return ((const uint8_t *)base)[i];
+    case 2: return ((const int16_t *)base)[i];
+    case 3: return ((const uint16_t *)base)[i];
+    case 4: return ((const int32_t *)base)[i];
+    case 5: return ((const uint32_t *)base)[i];
+    case 6: return ((const int64_t *)base)[i];
+    case 7: return ((const uint64_t *)base)[i];
+    case 8: return ((const float *)base)[i];
+    case 9: return ((const double *)base)[i];
+  }
+  return 0.0L;
+}
+
+/* Evaluate hook `kind`; returns the response widened to long double (exact
+ * for every FeatType).  `p` holds FeatType params. */
+long double or_feature(int kind, const or_image *im, int x, int y, const void *p)
+{
+  if (is_generic(kind)) return feat_channel_diff(kind, im, x, y, p, NULL);
   switch (kind) {
     case OR_FEAT_DEPTH_F32: return feat_depth_f32(im, x, y, (const float *)p);
     case OR_FEAT_DEPTH_U16: return feat_depth_u16(im, x, y, (const float *)p);
@@ -264,11 +396,11 @@ double or_feature(int kind, const or_image *im, int x, int y, const void *p)
   return 0.0;
 }
 
-/* feat <= thr in FeatType (hist_update.cl:154; predict.cl:75) */
-static int goes_left(int kind, double v, const void *thr, uint32_t t)
+/* feat <= thr in FeatType (hist_update.cl:154; predict.cl:75): both sides
+ * are exact in long double, so the comparison is the FeatType one. */
+static int goes_left(int kind, long double v, const void *thr, uint32_t t)
 {
-  if (feat_is_short(kind)) return (int16_t)v <= ((const int16_t *)thr)[t];
-  return (float)v <= ((const float *)thr)[t];
+  return v <= feat_ld(kind, thr, t);
 }
 
 /* C-ABI wrapper for tests: evaluate a hook at (x, y) on one image. */
@@ -276,7 +408,15 @@ double or_feature_at(int kind, const void *data, int w, int h, int nch, int x, i
                      const void *params)
 {
   or_image im = {data, w, h, nch};
-  return or_feature(kind, &im, x, y, params);
+  return (double)or_feature(kind, &im, x, y, params);
+}
+
+/* Generic hook at (x, y): the FeatType response written to out. */
+void or_feature_generic_at(int kind, const void *data, int w, int h, int nch, int x, int y, const void *params,
+                           void *out)
+{
+  or_image im = {data, w, h, nch};
+  feat_channel_diff(kind, &im, x, y, params, out);
 }
 
 /* ------------------------------------------------------------------ */
@@ -407,7 +547,11 @@ typedef struct {
   uint32_t leafthr;
 } or_params;
 
-static size_t img_elem_size(int kind) { return kind == OR_FEAT_DEPTH_F32 ? 4 : 2; }
+static size_t img_elem_size(int kind)
+{
+  if (is_generic(kind)) return tag_size(gen_img(kind));
+  return kind == OR_FEAT_DEPTH_F32 ? 4 : 2;
+}
 
 static or_image image_of(const or_tset *ts, uint32_t i)
 {
@@ -435,13 +579,13 @@ static int32_t route(int kind, const or_tree *tr, const or_image *im, int x, int
                      uint32_t steps)
 {
   int fd = feat_dim_of(kind);
-  size_t fsz = feat_is_short(kind) ? 2 : 4;
+  size_t fsz = feat_size(kind);
   int32_t n = 0;
   for (uint32_t s = 0; s < steps; s++) {
     int32_t lc = tr->left[n];
     if (lc < 0) break;
     const void *p = (const char *)tr->feats + (size_t)n * fd * fsz;
-    double v = or_feature(kind, im, x, y, p);
+    long double v = or_feature(kind, im, x, y, p);
     n = lc + (goes_left(kind, v, tr->thrs, (uint32_t)n) ? 0 : 1);
   }
   return n;
@@ -459,7 +603,7 @@ int or_hist_nodes(const or_tset *ts, const or_params *pr, uint32_t nC, uint32_t 
                   uint32_t *H)
 {
   int kind = ts->kind, fd = feat_dim_of(kind);
-  size_t fsz = feat_is_short(kind) ? 2 : 4;
+  size_t fsz = feat_size(kind);
   uint32_t nF = pr->nF, nT = pr->nT;
   size_t per = (size_t)nC * nT * nF;
   memset(H, 0, sizeof(uint32_t) * per * n_nodes);
@@ -474,7 +618,7 @@ int or_hist_nodes(const or_tset *ts, const or_params *pr, uint32_t nC, uint32_t 
   for (int64_t fi = 0; fi < (int64_t)nF * n_nodes; fi++) {
     uint32_t sl = (uint32_t)(fi / nF), f = (uint32_t)(fi % nF);
     uint32_t node = (uint32_t)nodes[sl];
-    unsigned char params[64], thr[512];
+    unsigned char params[64], thr[8 * 64];
     or_feature_params(kind, tree_id, node, f, pr->flo, pr->fhi, params);
     or_thresholds(kind, tree_id, node, f, nT, pr->thrlo, pr->thrhi, thr);
     /* this candidate's counts [c][t], private to the thread */
@@ -489,7 +633,7 @@ int or_hist_nodes(const or_tset *ts, const or_params *pr, uint32_t nC, uint32_t 
         uint32_t px = smp[s];
         int x = (int)(px % (uint32_t)im.w), y = (int)(px / (uint32_t)im.w);
         uint32_t c = (uint32_t)lab[px] - 1u;
-        double v = or_feature(kind, &im, x, y, params);
+        long double v = or_feature(kind, &im, x, y, params);
         for (uint32_t t = 0; t < nT; t++) loc[(size_t)c * nT + t] += (uint32_t)goes_left(kind, v, thr, t);
       }
     }
@@ -517,7 +661,7 @@ int or_train_tree(const or_tset *ts, const or_params *pr, uint32_t nC, const flo
                   uint32_t *node_k, float *node_gain, double *node_gain64)
 {
   int kind = ts->kind, fd = feat_dim_of(kind);
-  size_t fsz = feat_is_short(kind) ? 2 : 4;
+  size_t fsz = feat_size(kind);
   if (end_depth > depth || end_depth < 1) return -1;
   size_t N = ((size_t)1 << depth) - 1;
   uint32_t nF = pr->nF, nT = pr->nT;
@@ -595,7 +739,7 @@ int or_train_tree(const or_tset *ts, const or_params *pr, uint32_t nC, const flo
       }
       left[2 * n + 1] = -1; left[2 * n + 2] = -1;
       or_feature_params(kind, tree_id, n, bf, pr->flo, pr->fhi, (char *)feats + (size_t)n * fd * fsz);
-      unsigned char th[512];
+      unsigned char th[8 * 64];
       or_thresholds(kind, tree_id, n, bf, bt + 1, pr->thrlo, pr->thrhi, th);
       memcpy((char *)thrs + (size_t)n * fsz, th + (size_t)bt * fsz, fsz);
       left[n] = (int32_t)(2 * n + 1);                 /* :233 */

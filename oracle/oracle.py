@@ -22,6 +22,19 @@ NCH = {0: 1, 1: 1, 2: 4, 3: 1}
 
 VP = C.c_void_p
 
+# Generic kinds (the ChannelDiff hook of padenti_amd/examples/channel_diff.hpp)
+OR_GENERIC = 0x10000
+TAGS = [np.int8, np.uint8, np.int16, np.uint16, np.int32, np.uint32, np.int64, np.uint64, np.float32, np.float64]
+
+
+def generic_kind(img_dtype, nch, feat_dtype):
+    """Kind code of ChannelDiff<img_dtype, nch, feat_dtype> (registered in FD, FDT, IMGT, NCH)."""
+    it = [np.dtype(t) for t in TAGS].index(np.dtype(img_dtype))
+    ft = [np.dtype(t) for t in TAGS].index(np.dtype(feat_dtype))
+    kind = OR_GENERIC | (nch << 8) | (it << 4) | ft
+    FD[kind], FDT[kind], IMGT[kind], NCH[kind] = 4, TAGS[ft], TAGS[it], nch
+    return kind
+
 
 class OrTset(C.Structure):
     _fields_ = [("kind", C.c_int), ("n_images", C.c_uint32), ("nch", C.c_uint32), ("pixels", VP),
@@ -74,6 +87,7 @@ def lib():
         L.or_predict_posterior.argtypes = [C.c_int, VP, C.c_int, C.c_int, C.c_int, VP, C.c_uint32, VP, VP, VP,
                                            VP, VP, C.c_uint32, VP]
         L.or_num_threads.restype = C.c_int
+        L.or_feature_generic_at.argtypes = [C.c_int, VP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, VP, VP]
         _lib = L
     return _lib
 
@@ -125,6 +139,16 @@ def feature_at(kind, image, x, y, params):
     h, w = img.shape[0], img.shape[1]
     p = np.ascontiguousarray(params, dtype=FDT[kind])
     return lib().or_feature_at(kind, _p(img), w, h, NCH[kind], x, y, _p(p))
+
+
+def feature_generic_at(kind, image, x, y, params):
+    """ChannelDiff response at (x, y) as a FeatType scalar."""
+    img = np.ascontiguousarray(image, dtype=IMGT[kind])
+    h, w = img.shape[0], img.shape[1]
+    p = np.ascontiguousarray(params, dtype=FDT[kind])
+    out = np.zeros(1, dtype=FDT[kind])
+    lib().or_feature_generic_at(kind, _p(img), w, h, NCH[kind], x, y, _p(p), _p(out))
+    return out[0]
 
 
 def log2_det(x):

@@ -120,7 +120,8 @@ class Tree:
     left_children: -2 nonexistent, -1 leaf, 2i+1 split."""
 
     def __init__(self, tree_id, depth, feature_kind, n_classes):
-        _, _, fdt, fd = FEATURES[feature_kind]
+        # feature_kind: a built-in PD_FEAT_* or a Plugin (its FeatType and FeatDim)
+        _, _, fdt, fd = feature_kind.info if hasattr(feature_kind, "info") else FEATURES[feature_kind]
         self.id, self.depth, self.n_classes, self.kind = int(tree_id), int(depth), int(n_classes), feature_kind
         n = (1 << depth) - 1
         self.left_children = np.full(n, -2, dtype=np.int32)

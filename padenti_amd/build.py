@@ -21,6 +21,12 @@ LIBS = ["-lz"]  # PNG streams
 # example of a separately compiled feature hook (include/padenti/pd_plugin.hpp)
 PLUGIN_LIB = os.path.join(HERE, "libpd_example_plugin.so")
 PLUGIN_SOURCES = [os.path.join(HERE, "examples", "depth_plugin.hip")]
+# one hook written for any element type / channel count / FeatType, ten
+# instantiations (every FeatType, up to 16 channels) in two plugin libraries
+# (a plugin library holds one translation unit of the engine)
+GENERIC_LIBS = {os.path.join(HERE, "libpd_channel_diff_%s.so" % k): [os.path.join(HERE, "examples",
+                                                                               "channel_diff_%s.hip" % k)]
+                for k in ("a", "b")}
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HIP_FLAGS = [
@@ -79,8 +85,9 @@ def build(verbose: bool = False, plugin: bool = True) -> str:
     if plugin:
         # the two libraries compile independently: run them side by side
         from concurrent.futures import ThreadPoolExecutor
-        with ThreadPoolExecutor(2) as ex:
+        with ThreadPoolExecutor(2 + len(GENERIC_LIBS)) as ex:
             jobs = [ex.submit(_compile, SOURCES, LIB, verbose, LIBS), ex.submit(_compile, PLUGIN_SOURCES, PLUGIN_LIB, verbose)]
+            jobs += [ex.submit(_compile, src, lib, verbose) for lib, src in GENERIC_LIBS.items()]
             for j in jobs:
                 j.result()
         build_demo(verbose)
