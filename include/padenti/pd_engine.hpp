@@ -391,6 +391,12 @@ struct Trainer final : TrainerBase {
   DevBuf d_cand_key, d_cand_left, d_hlist;
   DevBuf d_tleft, d_tfeat, d_tthr, d_rslot_node, d_slot_next, d_cursor, d_ccount;
   DevBuf d_flags, d_rank, d_sbegin, d_scan_tmp;
+  // auto-context hooks: node-id maps of every image (two buffers), the
+  // pixels that are routed (labelled), map offsets, the tree's posteriors
+  DevBuf d_acmapA, d_acmapB, d_acsel, d_acoff, d_acblk, d_tpost;
+  uint64_t ac_px = 0;
+  uint32_t ac_blocks = 0;
+  AcArgs ac{};
   uint32_t cu_count = 256;
   uint32_t n_xcc = 1;
   size_t lds_limit = 65536;
@@ -493,6 +499,33 @@ struct Trainer final : TrainerBase {
       HIPCHK(hipStreamSynchronize(stream));
       HIPCHK(hipMemcpy(h_recs.data(), d_recs.p, s_total * sizeof(uint4), hipMemcpyDeviceToHost));
     }
+    if constexpr (uses_context<F>::value) {
+      // node maps packed back to back; routed pixels = labelled ones
+      std::vector<uint64_t> moff(ni);
+      std::vector<uint32_t> blk0(ni);
+      uint64_t t = 0, b = 0;
+      for (uint32_t i = 0; i < ni; i++) {
+        moff[i] = t;
+        blk0[i] = (uint32_t)b;
+        const uint64_t npx = (uint64_t)widths[i] * heights[i];
+        t += npx;
+        b += (npx + 255) / 256;
+      }
+      require(b < (1ull << 31), "too many pixels for the node maps");
+      std::vector<uint8_t> sel(t);
+      for (uint32_t i = 0; i < ni; i++)
+        std::memcpy(sel.data() + moff[i], labels + lab_off[i], (size_t)widths[i] * heights[i]);
+      d_acsel.ensure(std::max<uint64_t>(t, 1));
+      HIPCHK(hipMemcpy(d_acsel.p, sel.data(), t, hipMemcpyHostToDevice));
+      d_acoff.ensure(ni * sizeof(uint64_t));
+      HIPCHK(hipMemcpy(d_acoff.p, moff.data(), ni * sizeof(uint64_t), hipMemcpyHostToDevice));
+      d_acblk.ensure(ni * sizeof(uint32_t));
+      HIPCHK(hipMemcpy(d_acblk.p, blk0.data(), ni * sizeof(uint32_t), hipMemcpyHostToDevice));
+      d_acmapA.ensure(std::max<uint64_t>(t, 1) * sizeof(int32_t));
+      d_acmapB.ensure(std::max<uint64_t>(t, 1) * sizeof(int32_t));
+      ac_px = t;
+      ac_blocks = (uint32_t)b;
+    }
     n_images = ni;
     S = s_total;
   }
@@ -533,7 +566,7 @@ struct Trainer final : TrainerBase {
                        d_imgs.as<ImgDesc>(), act, d_items.as<HistItem>(), item_base, d_slot_node.as<int32_t>(),
                        d_slot_mode.as<uint8_t>(), d_slot_hidx.as<int32_t>(), d_tot.as<uint32_t>(), bnd, tree, nF,
                        nT, nC, d_H.as<uint32_t>(), d_cand_key.as<unsigned long long>(), d_cand_left.as<uint32_t>(),
-                       d_map_cur);
+                       d_map_cur, ac);
     HIPCHK(hipGetLastError());
   }
 
@@ -960,6 +993,22 @@ struct Trainer final : TrainerBase {
     uint4 *act = d_actA.as<uint4>(), *act_next = d_actB.as<uint4>();
     uint32_t *aslot = d_slotA.as<uint32_t>(), *aslot_next = d_slotB.as<uint32_t>();
     uint64_t n_act = S;
+    int32_t *acmap = nullptr, *acmap_next = nullptr;
+    if constexpr (uses_context<F>::value) {
+      // the device tree mirrors the host one: all -2, root posterior = priors;
+      // every labelled pixel starts at the root
+      d_tpost.ensure(N * nC * sizeof(float));
+      hipLaunchKernelGGL(k_fill_i32, dim3((uint32_t)std::min<size_t>((N + 255) / 256, 65535)), dim3(256), 0, stream,
+                         d_tleft.as<int32_t>(), (uint64_t)N, -2);
+      HIPCHK(hipMemsetAsync(d_tpost.p, 0, N * nC * sizeof(float), stream));
+      HIPCHK(hipMemcpyAsync(d_tpost.p, prior, nC * sizeof(float), hipMemcpyHostToDevice, stream));
+      acmap = d_acmapA.as<int32_t>();
+      acmap_next = d_acmapB.as<int32_t>();
+      hipLaunchKernelGGL(k_ac_map_init, dim3((uint32_t)std::min<uint64_t>((ac_px + 255) / 256, 65535)), dim3(256), 0,
+                         stream, d_acsel.as<uint8_t>(), ac_px, acmap);
+      HIPCHK(hipGetLastError());
+      ac = AcArgs{d_tleft.as<int32_t>(), d_tpost.as<float>(), nC, acmap, d_acoff.as<uint64_t>()};
+    }
 
     std::vector<SlotInfo> slots{SlotInfo{0, 0u, (uint32_t)S, S > thr}};
     Event e3, e4;
@@ -1049,6 +1098,21 @@ struct Trainer final : TrainerBase {
       HIPCHK(hipMemcpyAsync(d_tfeat.as<T>() + lv0 * FD, feats + lv0 * FD, lvn * FD * sizeof(T),
                             hipMemcpyHostToDevice, stream));
       HIPCHK(hipMemcpyAsync(d_tthr.as<T>() + lv0, thrs + lv0, lvn * sizeof(T), hipMemcpyHostToDevice, stream));
+      if constexpr (uses_context<F>::value) {
+        // the children (next level) as the hook will read them: leftChild -1
+        // and their posteriors (the reference's device copy reads the
+        // posteriors from the wrong offset, _learn_best_featthr.hpp:270-274)
+        const size_t c0 = lv0 + lvn, cn = std::min(2 * lvn, N - c0);
+        HIPCHK(hipMemcpyAsync(d_tleft.as<int32_t>() + c0, left + c0, cn * sizeof(int32_t), hipMemcpyHostToDevice,
+                              stream));
+        HIPCHK(hipMemcpyAsync(d_tpost.as<float>() + c0 * nC, post + c0 * nC, cn * nC * sizeof(float),
+                              hipMemcpyHostToDevice, stream));
+        // every routed pixel one level down (context: the map before the step)
+        hipLaunchKernelGGL((k_ac_route_map<F>), dim3(ac_blocks), dim3(256), 0, stream, d_pixels.as<I>(),
+                           d_imgs.as<ImgDesc>(), n_images, d_acblk.as<uint32_t>(), d_tfeat.as<T>(), d_tthr.as<T>(),
+                           ac, acmap_next);
+        HIPCHK(hipGetLastError());
+      }
       // next frontier (children with more than perLeafSamplesThr samples,
       // _impl_init.hpp:343-355), in node order
       std::vector<SlotInfo> next;
@@ -1095,7 +1159,7 @@ struct Trainer final : TrainerBase {
       const uint32_t nb = (uint32_t)((n_act + 1 + 255) / 256);
       hipLaunchKernelGGL((k_route_flags<F>), dim3(nb), dim3(256), 0, stream, d_pixels.as<I>(), d_imgs.as<ImgDesc>(),
                          act, aslot, (uint32_t)n_act, d_rslot_node.as<int32_t>(), d_tleft.as<int32_t>(),
-                         d_tfeat.as<T>(), d_tthr.as<T>(), d_flags.as<unsigned long long>());
+                         d_tfeat.as<T>(), d_tthr.as<T>(), d_flags.as<unsigned long long>(), ac);
       HIPCHK(hipGetLastError());
       HIPCHK(hipcub::DeviceScan::ExclusiveSum(d_scan_tmp.p, tmp_bytes, d_flags.as<unsigned long long>(),
                                               d_rank.as<unsigned long long>(), (int)(n_act + 1), stream));
@@ -1124,6 +1188,10 @@ struct Trainer final : TrainerBase {
       }
       std::swap(act, act_next);
       std::swap(aslot, aslot_next);
+      if constexpr (uses_context<F>::value) {
+        std::swap(acmap, acmap_next);
+        ac.maps = acmap;
+      }
       n_act = run;
       slots.swap(next);
       st.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
@@ -1137,6 +1205,8 @@ struct Trainer final : TrainerBase {
   double time_level(const pd_train_params &P, uint32_t tree_id, uint32_t n_nodes, int reps) override
   {
     if (!S) throw StateError("pd_trainer_time_level before pd_trainer_set_data");
+    if constexpr (uses_context<F>::value)
+      throw ArgError("this single-step entry point has no tree state for an auto-context hook");
     check_params(P, FD, sizeof(T) == 2);
     require(n_nodes >= 1 && n_nodes <= S && reps >= 1, "bad level shape");
     set_device(device);
@@ -1168,6 +1238,8 @@ struct Trainer final : TrainerBase {
                   const int32_t *sample_node, uint32_t *H_out) override
   {
     if (!S) throw StateError("pd_trainer_hist_nodes before pd_trainer_set_data");
+    if constexpr (uses_context<F>::value)
+      throw ArgError("this single-step entry point has no tree state for an auto-context hook");
     check_params(P, FD, sizeof(T) == 2);
     require(n_nodes >= 1 && nodes && sample_node && H_out, "bad hist_nodes arguments");
     set_device(device);
@@ -1249,6 +1321,13 @@ struct Forest final : ForestBase {
   uint32_t max_depth = 1;
   DevBuf d_nodes, d_leaf_post, d_leaf_id, d_tree_off, d_leaf_off;
   DevBuf d_img, d_mask, d_out, d_leaf, d_frames;
+  // auto-context hooks: the trees' breadth-first arrays (the hook reads them)
+  std::vector<int32_t> h_ac_left;
+  std::vector<T> h_ac_feat, h_ac_thr;
+  std::vector<float> h_ac_post;
+  std::vector<uint64_t> h_ac_off;   // first node of each tree
+  DevBuf d_ac_left, d_ac_feat, d_ac_thr, d_ac_post, d_ac_mapA, d_ac_mapB, d_ac_img, d_ac_off0, d_ac_blk0;
+  bool ac_dirty = false;
   // captured graph for predict_frames (keyed by the launch arguments)
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
@@ -1283,6 +1362,14 @@ struct Forest final : ForestBase {
     // compact the heap layout (cl_classifier_impl.hpp:153-168 uploads the
     // whole 2^D - 1 arrays): breadth-first walk of the existing nodes, a
     // split node's children placed side by side
+    if constexpr (uses_context<F>::value) {
+      h_ac_off.push_back(h_ac_left.size());
+      h_ac_left.insert(h_ac_left.end(), left, left + N);
+      h_ac_feat.insert(h_ac_feat.end(), feats, feats + N * F::FeatDim);
+      h_ac_thr.insert(h_ac_thr.end(), thrs, thrs + N);
+      h_ac_post.insert(h_ac_post.end(), posts, posts + N * nC);
+      ac_dirty = true;
+    }
     const uint32_t base = (uint32_t)h_nodes.size(), leaf_base = (uint32_t)h_leaf_id.size();
     require((uint64_t)h_nodes.size() + N < (1ull << 31), "forest too large");
     std::vector<int64_t> order{0};   // original ids in compact order
@@ -1353,8 +1440,98 @@ struct Forest final : ForestBase {
     else launch<64>(blocks, pix, frames, n_frames, masks, tree0, ntr, steps, out_post, out_leaf);
   }
 
+  // Auto-context prediction of one frame resident on the device: per tree,
+  // depth - 1 passes over the frame's routed pixels (a node-id map, the hook
+  // reading the map of the pass before), then the leaf ids / the posterior
+  // sum in tree order (cl_classifier_impl.hpp:356-557 runs the same passes).
+  void predict_ac_frame(const I *d_pix, uint32_t w, uint32_t h, const uint8_t *d_mask, int tree_index,
+                        int32_t *d_leaves, float *d_post)
+  {
+    if (ac_dirty) {
+      auto up = [&](DevBuf &b, const void *src, size_t bytes) {
+        b.ensure(std::max<size_t>(bytes, 16));
+        HIPCHK(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, stream));
+      };
+      up(d_ac_left, h_ac_left.data(), h_ac_left.size() * sizeof(int32_t));
+      up(d_ac_feat, h_ac_feat.data(), h_ac_feat.size() * sizeof(T));
+      up(d_ac_thr, h_ac_thr.data(), h_ac_thr.size() * sizeof(T));
+      up(d_ac_post, h_ac_post.data(), h_ac_post.size() * sizeof(float));
+      ac_dirty = false;
+    }
+    const uint32_t npx = w * h;
+    const uint32_t blocks = (npx + 255) / 256;
+    d_ac_mapA.ensure((size_t)npx * sizeof(int32_t));
+    d_ac_mapB.ensure((size_t)npx * sizeof(int32_t));
+    const ImgDesc idsc{0, w, h};
+    const uint64_t zero64 = 0;
+    const uint32_t zero32 = 0;
+    d_ac_img.ensure(sizeof(ImgDesc));
+    d_ac_off0.ensure(sizeof(uint64_t));
+    d_ac_blk0.ensure(sizeof(uint32_t));
+    HIPCHK(hipMemcpyAsync(d_ac_img.p, &idsc, sizeof(ImgDesc), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(d_ac_off0.p, &zero64, sizeof(uint64_t), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(d_ac_blk0.p, &zero32, sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+    const int t0 = tree_index >= 0 ? tree_index : 0;
+    const int t1 = tree_index >= 0 ? tree_index + 1 : (int)depths.size();
+    for (int t = t0; t < t1; t++) {
+      int32_t *a = d_ac_mapA.as<int32_t>(), *b = d_ac_mapB.as<int32_t>();
+      hipLaunchKernelGGL(k_ac_map_init, dim3(std::max<uint32_t>(1, std::min<uint32_t>(blocks, 65535))), dim3(256), 0,
+                         stream, d_mask, (uint64_t)npx, a);
+      const uint64_t o = h_ac_off[t];
+      for (uint32_t s = 0; s + 1 < depths[t]; s++) {
+        const AcArgs acx{d_ac_left.as<int32_t>() + o, d_ac_post.as<float>() + o * nC, nC, a, d_ac_off0.as<uint64_t>()};
+        hipLaunchKernelGGL((k_ac_route_map<F>), dim3(blocks), dim3(256), 0, stream, d_pix, d_ac_img.as<ImgDesc>(), 1u,
+                           d_ac_blk0.as<uint32_t>(), d_ac_feat.as<T>() + o * F::FeatDim, d_ac_thr.as<T>() + o, acx, b);
+        std::swap(a, b);
+      }
+      HIPCHK(hipGetLastError());
+      if (d_leaves && tree_index >= 0)
+        HIPCHK(hipMemcpyAsync(d_leaves, a, (size_t)npx * sizeof(int32_t), hipMemcpyDeviceToDevice, stream));
+      if (d_post)
+        hipLaunchKernelGGL(k_ac_post_add, dim3(blocks), dim3(256), 0, stream, a, npx, d_ac_post.as<float>() + o * nC,
+                           nC, t == t0 ? 1 : 0, d_post);
+      if (d_post && t + 1 == t1)
+        hipLaunchKernelGGL(k_ac_post_finish, dim3(blocks), dim3(256), 0, stream, a, npx, nC,
+                           (float)depths.size(), d_post);
+    }
+    HIPCHK(hipGetLastError());
+  }
+
   void predict_host(int tree_index, const void *image, uint32_t w, uint32_t h, const uint8_t *mask,
                     int32_t *leaves, float *post) override
+  {
+    if constexpr (uses_context<F>::value) {
+      require(!depths.empty(), "forest has no trees");
+      require(image && w >= 1 && h >= 1 && w < 65536 && h < 65536, "bad image");
+      set_device(device);
+      const size_t npx = (size_t)w * h;
+      d_img.ensure(npx * F::nChannels * sizeof(I));
+      HIPCHK(hipMemcpyAsync(d_img.p, image, npx * F::nChannels * sizeof(I), hipMemcpyHostToDevice, stream));
+      if (mask) {
+        d_mask.ensure(npx);
+        HIPCHK(hipMemcpyAsync(d_mask.p, mask, npx, hipMemcpyHostToDevice, stream));
+      }
+      if (leaves) {
+        require(tree_index >= 0 && tree_index < (int)depths.size(), "tree index out of range");
+        d_leaf.ensure(npx * sizeof(int32_t));
+        predict_ac_frame(d_img.as<I>(), w, h, mask ? d_mask.as<uint8_t>() : nullptr, tree_index,
+                         d_leaf.as<int32_t>(), nullptr);
+        HIPCHK(hipMemcpyAsync(leaves, d_leaf.p, npx * sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+      }
+      if (post) {
+        d_out.ensure(npx * nC * sizeof(float));
+        predict_ac_frame(d_img.as<I>(), w, h, mask ? d_mask.as<uint8_t>() : nullptr, -1, nullptr,
+                         d_out.as<float>());
+        HIPCHK(hipMemcpyAsync(post, d_out.p, npx * nC * sizeof(float), hipMemcpyDeviceToHost, stream));
+      }
+      HIPCHK(hipStreamSynchronize(stream));
+      return;
+    }
+    predict_host_plain(tree_index, image, w, h, mask, leaves, post);
+  }
+
+  void predict_host_plain(int tree_index, const void *image, uint32_t w, uint32_t h, const uint8_t *mask,
+                          int32_t *leaves, float *post)
   {
     require(!depths.empty(), "forest has no trees");
     require(image && w >= 1 && h >= 1 && w < 65536 && h < 65536, "bad image");
@@ -1401,6 +1578,21 @@ struct Forest final : ForestBase {
     require(!d_leaves || leaf_off, "leaf_offsets missing");
     require(!d_masks || mask_off, "mask_offsets missing");
     set_device(device);
+    if constexpr (uses_context<F>::value) {
+      require(!use_graph, "auto-context hooks predict frame by frame (no graph capture)");
+      require(!d_leaves, "auto-context hooks: leaf ids per tree through pd_forest_predict_leaves");
+      Event a, b;
+      HIPCHK(hipEventRecord(a.e, stream));
+      for (uint32_t i = 0; i < n_frames; i++) {
+        require(widths[i] >= 1 && heights[i] >= 1 && widths[i] < 65536 && heights[i] < 65536, "bad frame size");
+        predict_ac_frame((const I *)d_pixels + img_off[i], widths[i], heights[i], d_masks ? d_masks + mask_off[i] : nullptr,
+                         -1, nullptr, d_post + post_off[i]);
+      }
+      HIPCHK(hipEventRecord(b.e, stream));
+      HIPCHK(hipEventSynchronize(b.e));
+      if (kernel_ms) *kernel_ms = (float)elapsed_ms(a, b);
+      return;
+    }
     std::vector<FrameDesc> fr(n_frames);
     uint32_t blocks = 0;
     for (uint32_t i = 0; i < n_frames; i++) {

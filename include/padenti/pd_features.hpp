@@ -269,6 +269,65 @@ PD_HD P unpack_prep(uint32_t z, uint32_t w)
   return v;
 }
 
+// ---------------------------------------------------------------------------
+// Auto-context hooks (the reference passes treeLeftChildren, treePosteriors
+// and the image of node ids to every hook call, hist_update.cl:122-124,
+// predict.cl:70-74).  A functor opts in with
+//   static constexpr bool kAutoContext = true;
+//   static FeatType eval_ctx(view, x, y, Prep, const FeatType *params,
+//                            const AutoContext &ctx);
+// and may then read the tree being trained (or predicted with) and the node
+// id of any pixel of its image at the current depth (-1 outside the image
+// and for pixels that are not routed: unlabelled in training, masked out in
+// prediction).
+// ---------------------------------------------------------------------------
+struct AutoContext {
+  const int32_t *left;      // left children of the tree (breadth-first heap)
+  const float *post;        // posteriors [node][n_classes]
+  uint32_t n_classes;
+  const int32_t *node_ids;  // this image's node id per pixel, row-major
+  int w, h;
+  PD_HD int32_t node_at(int x, int y) const
+  {
+    return (x < 0 || x >= w || y < 0 || y >= h) ? -1 : node_ids[(size_t)y * (size_t)w + (size_t)x];
+  }
+  PD_HD float posterior(int32_t node, uint32_t c) const
+  {
+    return (node < 0 || c >= n_classes) ? 0.0f : post[(size_t)node * n_classes + c];
+  }
+};
+
+// Kernel-side view over all images: node maps packed back to back, image i's
+// map at map_off[i].  All pointers are null for hooks without context.
+struct AcArgs {
+  const int32_t *left;
+  const float *post;
+  uint32_t n_classes;
+  const int32_t *maps;
+  const uint64_t *map_off;
+};
+
+template <class F, class = void>
+struct uses_context : std::false_type {};
+template <class F>
+struct uses_context<F, std::void_t<decltype(F::kAutoContext)>> : std::integral_constant<bool, F::kAutoContext> {};
+
+// The hook call of every kernel: eval_prep, or eval_ctx with image `img`'s context.
+template <class F>
+PD_HD typename F::FeatType eval_hook(const ImageView<typename F::ImgType, F::nChannels> &view, int x, int y,
+                                     typename F::Prep prep, const typename F::FeatType *p, const AcArgs &ac,
+                                     uint32_t img)
+{
+  if constexpr (uses_context<F>::value) {
+    const AutoContext ctx{ac.left, ac.post, ac.n_classes, ac.maps + ac.map_off[img], view.w, view.h};
+    return F::eval_ctx(view, x, y, prep, p, ctx);
+  } else {
+    (void)ac;
+    (void)img;
+    return F::eval_prep(view, x, y, prep, p);
+  }
+}
+
 // Threshold padding value: compares "left" against every response, so a
 // padded slot never counts as "right" for a finite response (+inf for the
 // floating types, the largest value for the integer ones).

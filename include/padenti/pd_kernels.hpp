@@ -122,14 +122,14 @@ __device__ __forceinline__ void hist_sample(const typename F::ImgType *__restric
                                             const ImgDesc *__restrict__ imgs, uint4 r,
                                             const typename F::FeatType (&p)[F::FeatDim],
                                             const typename F::FeatType *sth, uint32_t nT, bool fv,
-                                            uint32_t lane, uint32_t *bins)
+                                            uint32_t lane, uint32_t *bins, const AcArgs &ac)
 {
   using T = typename F::FeatType;
   constexpr int RW = rank_words<NTP>();
   constexpr int NS = sorted_slots<NTP>();
   const ImgDesc d = imgs[rec_img(r)];
   ImageView<typename F::ImgType, F::nChannels> view{pixels + d.off, (int)d.w, (int)d.h};
-  const T v = F::eval_prep(view, rec_x(r), rec_y(r), unpack_prep<typename F::Prep>(r.z, r.w), p);
+  const T v = eval_hook<F>(view, rec_x(r), rec_y(r), unpack_prep<typename F::Prep>(r.z, r.w), p, ac, rec_img(r));
   // binary search carried as the byte offset of slot `pos` in the lane's
   // column (slot stride 64 * sizeof(T)); the step offsets fold into the LDS
   // read's immediate
@@ -170,7 +170,7 @@ __global__ __launch_bounds__(NWAVES * 64) void k_hist(
     const int32_t *__restrict__ slot_node, const uint8_t *__restrict__ slot_mode,
     const int32_t *__restrict__ slot_hidx, const uint32_t *__restrict__ tot, Bounds<F> bnd, uint32_t tree,
     uint32_t nF, uint32_t nT, uint32_t nC, uint32_t *__restrict__ H, unsigned long long *__restrict__ cand_key,
-    uint32_t *__restrict__ cand_left, const uint32_t *__restrict__ cand_map)
+    uint32_t *__restrict__ cand_left, const uint32_t *__restrict__ cand_map, AcArgs ac)
 {
   using T = typename F::FeatType;
   constexpr int FD = F::FeatDim;
@@ -251,10 +251,10 @@ __global__ __launch_bounds__(NWAVES * 64) void k_hist(
   for (; i + NW < end; i += 2 * NW) {
     const uint4 ra = recs[i];
     const uint4 rb = recs[i + NW];
-    hist_sample<F, NTP>(pixels, imgs, ra, p, sth, nT, fv, lane, bins);
-    hist_sample<F, NTP>(pixels, imgs, rb, p, sth, nT, fv, lane, bins);
+    hist_sample<F, NTP>(pixels, imgs, ra, p, sth, nT, fv, lane, bins, ac);
+    hist_sample<F, NTP>(pixels, imgs, rb, p, sth, nT, fv, lane, bins, ac);
   }
-  if (i < end) hist_sample<F, NTP>(pixels, imgs, recs[i], p, sth, nT, fv, lane, bins);
+  if (i < end) hist_sample<F, NTP>(pixels, imgs, recs[i], p, sth, nT, fv, lane, bins, ac);
   __syncthreads();
 
   // Flush: per class (classes strided over waves) turn this lane's rank bins
@@ -589,7 +589,7 @@ __global__ __launch_bounds__(256) void k_route_flags(const typename F::ImgType *
                                                      const int32_t *__restrict__ tree_left,
                                                      const typename F::FeatType *__restrict__ tree_feat,
                                                      const typename F::FeatType *__restrict__ tree_thr,
-                                                     unsigned long long *__restrict__ flags)
+                                                     unsigned long long *__restrict__ flags, AcArgs ac)
 {
   using T = typename F::FeatType;
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -605,7 +605,7 @@ __global__ __launch_bounds__(256) void k_route_flags(const typename F::ImgType *
       T p[F::FeatDim];
 #pragma unroll
       for (int j = 0; j < F::FeatDim; j++) p[j] = tree_feat[(size_t)node * F::FeatDim + j];
-      const T v = F::eval_prep(view, rec_x(r), rec_y(r), unpack_prep<typename F::Prep>(r.z, r.w), p);
+      const T v = eval_hook<F>(view, rec_x(r), rec_y(r), unpack_prep<typename F::Prep>(r.z, r.w), p, ac, rec_img(r));
       fl = (v <= tree_thr[node]) ? 1ull : (1ull << 32);
     }
   }
@@ -721,7 +721,8 @@ __global__ __launch_bounds__(256) void k_predict(
           leaf = -1 - rec.left;
           break;
         }
-        const T v = F::eval_prep(view, x, y, prep, rec.p);
+        // (auto-context hooks never come here: they predict through node maps)
+        const T v = eval_hook<F>(view, x, y, prep, rec.p, AcArgs{}, 0u);
         n = rec.left + ((v <= rec.thr) ? 0 : 1);
       }
       const uint32_t li = leaf_off[t] + (uint32_t)leaf;
@@ -749,6 +750,89 @@ __global__ __launch_bounds__(256) void k_predict(
     for (int c = 0; c < NCM; c++)
       if ((uint32_t)c < nC) out_post[fd.post_off + (size_t)c * npx + px] = on ? acc[c] / nt : 0.0f;
   }
+}
+
+// ---------------------------------------------------------------------------
+// Node-id maps of auto-context hooks.  One pass moves every routed pixel of
+// a node that has children one level down: map_out = child by the hook
+// evaluated with the context of map_in (the maps of all images packed,
+// image i at ac.map_off[i]; a block per 256 pixels of one image, block0[i] =
+// first block of image i).  Pixels at -1 or at a leaf keep their value.
+// ---------------------------------------------------------------------------
+template <class F>
+__global__ __launch_bounds__(256) void k_ac_route_map(const typename F::ImgType *__restrict__ pixels,
+                                                      const ImgDesc *__restrict__ imgs, uint32_t n_images,
+                                                      const uint32_t *__restrict__ block0,
+                                                      const typename F::FeatType *__restrict__ tree_feat,
+                                                      const typename F::FeatType *__restrict__ tree_thr, AcArgs ac,
+                                                      int32_t *__restrict__ map_out)
+{
+  using T = typename F::FeatType;
+  uint32_t lo = 0, hi = n_images;  // image of this block: last with block0 <= blockIdx.x (uniform)
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (block0[mid] <= blockIdx.x) lo = mid;
+    else hi = mid;
+  }
+  const ImgDesc d = imgs[lo];
+  const uint32_t npx = d.w * d.h;
+  const uint32_t px = (blockIdx.x - block0[lo]) * 256u + threadIdx.x;
+  if (px >= npx) return;
+  const uint64_t mo = ac.map_off[lo];
+  const int32_t n = ac.maps[mo + px];
+  int32_t out = n;
+  if (n >= 0) {
+    const int32_t lc = ac.left[n];
+    if (lc >= 0) {
+      const int x = (int)(px % d.w), y = (int)(px / d.w);
+      ImageView<typename F::ImgType, F::nChannels> view{pixels + d.off, (int)d.w, (int)d.h};
+      T p[F::FeatDim];
+#pragma unroll
+      for (int j = 0; j < F::FeatDim; j++) p[j] = tree_feat[(size_t)n * F::FeatDim + j];
+      const T v = eval_hook<F>(view, x, y, F::prepare(view, x, y), p, ac, lo);
+      out = lc + ((v <= tree_thr[n]) ? 0 : 1);
+    }
+  }
+  map_out[mo + px] = out;
+}
+
+// map[i] = 0 (the root) where the pixel is routed (labels[i] != 0 or mask[i]
+// != 0; every pixel when sel is null), else -1.
+__global__ __launch_bounds__(256) void k_ac_map_init(const uint8_t *__restrict__ sel, uint64_t n,
+                                                     int32_t *__restrict__ map)
+{
+  for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256u)
+    map[i] = (sel && !sel[i]) ? -1 : 0;
+}
+
+// Forest posterior over node maps: out[c][i] (+)= post[map[i]][c] in tree
+// order (first: assign), then k_ac_post_finish divides by the tree count
+// (masked-out pixels: 0), the order of k_predict's sum.
+__global__ __launch_bounds__(256) void k_ac_post_add(const int32_t *__restrict__ map, uint32_t npx,
+                                                     const float *__restrict__ post, uint32_t nC, int first,
+                                                     float *__restrict__ out)
+{
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= npx) return;
+  const int32_t n = map[i];
+  for (uint32_t c = 0; c < nC; c++) {
+    const float v = n >= 0 ? post[(size_t)n * nC + c] : 0.0f;
+    out[(size_t)c * npx + i] = first ? v : out[(size_t)c * npx + i] + v;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_ac_post_finish(const int32_t *__restrict__ map, uint32_t npx, uint32_t nC,
+                                                        float n_trees, float *__restrict__ out)
+{
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= npx) return;
+  const bool on = map[i] >= 0;
+  for (uint32_t c = 0; c < nC; c++) out[(size_t)c * npx + i] = on ? out[(size_t)c * npx + i] / n_trees : 0.0f;
+}
+
+__global__ __launch_bounds__(256) void k_fill_i32(int32_t *__restrict__ p, uint64_t n, int32_t v)
+{
+  for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256u) p[i] = v;
 }
 
 // ---------------------------------------------------------------------------

@@ -85,7 +85,8 @@ enum {
   OR_FEAT_DEPTH_F32 = 0,   /* float metres, FeatType float, FD 4 */
   OR_FEAT_DEPTH_U16 = 1,   /* uint16 mm, FeatType float, FD 4 */
   OR_FEAT_RGBD_U16X4 = 2,  /* uint16 x4 (R,G,B,D mm), FeatType float, FD 6 */
-  OR_FEAT_DEMO_S16 = 3     /* uint16 mm, FeatType short, FD 2 (test/feature.cl) */
+  OR_FEAT_DEMO_S16 = 3,    /* uint16 mm, FeatType short, FD 2 (test/feature.cl) */
+  OR_FEAT_AC_DEPTH = 4     /* auto-context: float depth + posterior of an offset pixel's node, FD 4 */
 };
 
 /* Generic kinds: the ChannelDiff hook (padenti_amd/examples/channel_diff.hpp)
@@ -111,6 +112,7 @@ static int feat_dim_of(int kind)
     case OR_FEAT_DEPTH_U16: return 4;
     case OR_FEAT_RGBD_U16X4: return 6;
     case OR_FEAT_DEMO_S16: return 2;
+    case OR_FEAT_AC_DEPTH: return 4;
   }
   return -1;
 }
@@ -382,6 +384,34 @@ static long double feat_ld(int kind, const void *base, uint32_t i)
   return 0.0L;
 }
 
+/* Context of an auto-context hook: the tree as it stands, the node-id image
+ * of the hook's image (-1: pixel not routed) (hist_update.cl:122-124,
+ * predict.cl:70-74). */
+typedef struct {
+  const int32_t *left;
+  const float *post;
+  uint32_t nC;
+  const int32_t *map;
+} or_ctx;
+
+/* padenti_amd/examples/context_plugin.hip:
+ * f = (I(x + ox, y + oy) - I(x, y)) + w * P(c | node of (x + ox, y + oy)),
+ * params (ox, oy, c, w), offsets truncated, c mod nC, outside: I = 10, P = 0. */
+static float feat_ac_depth(const or_image *im, int x, int y, const float *p, const or_ctx *ctx)
+{
+  int ox = x + (int)p[0], oy = y + (int)p[1];
+  int in = ox >= 0 && ox < im->w && oy >= 0 && oy < im->h;
+  float a = in ? px_f32(im, ox, oy, 0) : 10.0f;
+  uint32_t c = (uint32_t)p[2] % ctx->nC;
+  int32_t n = in ? ctx->map[(size_t)oy * im->w + ox] : -1;
+  float q = n >= 0 ? ctx->post[(size_t)n * ctx->nC + c] : 0.0f;
+  float d = a - px_f32(im, x, y, 0);
+  float wq = p[3] * q;
+  return d + wq;
+}
+
+static long double or_feature_ctx(int kind, const or_image *im, int x, int y, const void *p, const or_ctx *ctx);
+
 /* Evaluate hook `kind`; returns the response widened to long double (exact
  * for every FeatType).  `p` holds FeatType params. */
 long double or_feature(int kind, const or_image *im, int x, int y, const void *p)
@@ -394,6 +424,12 @@ long double or_feature(int kind, const or_image *im, int x, int y, const void *p
     case OR_FEAT_DEMO_S16: return feat_demo_s16(im, x, y, (const int16_t *)p);
   }
   return 0.0;
+}
+
+static long double or_feature_ctx(int kind, const or_image *im, int x, int y, const void *p, const or_ctx *ctx)
+{
+  if (kind == OR_FEAT_AC_DEPTH) return feat_ac_depth(im, x, y, (const float *)p, ctx);
+  return or_feature(kind, im, x, y, p);
 }
 
 /* feat <= thr in FeatType (hist_update.cl:154; predict.cl:75): both sides
@@ -409,6 +445,15 @@ double or_feature_at(int kind, const void *data, int w, int h, int nch, int x, i
 {
   or_image im = {data, w, h, nch};
   return (double)or_feature(kind, &im, x, y, params);
+}
+
+/* The auto-context hook at (x, y) with an explicit context (tests). */
+float or_feature_ac_at(const float *img, int w, int h, int x, int y, const float *params, const int32_t *left,
+                       const float *post, uint32_t nC, const int32_t *map)
+{
+  or_image im = {img, w, h, 1};
+  or_ctx cx = {left, post, nC, map};
+  return feat_ac_depth(&im, x, y, params, &cx);
 }
 
 /* Generic hook at (x, y): the FeatType response written to out. */
@@ -550,7 +595,7 @@ typedef struct {
 static size_t img_elem_size(int kind)
 {
   if (is_generic(kind)) return tag_size(gen_img(kind));
-  return kind == OR_FEAT_DEPTH_F32 ? 4 : 2;
+  return (kind == OR_FEAT_DEPTH_F32 || kind == OR_FEAT_AC_DEPTH) ? 4 : 2;
 }
 
 static or_image image_of(const or_tset *ts, uint32_t i)
@@ -598,9 +643,22 @@ static int32_t route(int kind, const or_tree *tr, const or_image *im, int x, int
 /* cl_tree_trainer_impl_traverse_ts.hpp:518-617.  `sample_node` gives  */
 /* the node of every sample (packed order).  OpenMP over candidates.   */
 /* ------------------------------------------------------------------ */
+static int hist_nodes_ctx(const or_tset *ts, const or_params *pr, uint32_t nC, uint32_t tree_id,
+                          uint32_t n_nodes, const int32_t *nodes, const int32_t *sample_node, uint32_t *H,
+                          const int32_t *tleft, const float *tpost, int32_t *const *maps);
+
 int or_hist_nodes(const or_tset *ts, const or_params *pr, uint32_t nC, uint32_t tree_id,
                   uint32_t n_nodes, const int32_t *nodes, const int32_t *sample_node,
                   uint32_t *H)
+{
+  return hist_nodes_ctx(ts, pr, nC, tree_id, n_nodes, nodes, sample_node, H, NULL, NULL, NULL);
+}
+
+/* The same with the context of auto-context hooks: tree arrays and each
+ * image's node-id map. */
+static int hist_nodes_ctx(const or_tset *ts, const or_params *pr, uint32_t nC, uint32_t tree_id,
+                          uint32_t n_nodes, const int32_t *nodes, const int32_t *sample_node, uint32_t *H,
+                          const int32_t *tleft, const float *tpost, int32_t *const *maps)
 {
   int kind = ts->kind, fd = feat_dim_of(kind);
   size_t fsz = feat_size(kind);
@@ -633,7 +691,8 @@ int or_hist_nodes(const or_tset *ts, const or_params *pr, uint32_t nC, uint32_t 
         uint32_t px = smp[s];
         int x = (int)(px % (uint32_t)im.w), y = (int)(px / (uint32_t)im.w);
         uint32_t c = (uint32_t)lab[px] - 1u;
-        long double v = or_feature(kind, &im, x, y, params);
+        or_ctx cx = {tleft, tpost, nC, maps ? maps[i] : NULL};
+        long double v = or_feature_ctx(kind, &im, x, y, params, &cx);
         for (uint32_t t = 0; t < nT; t++) loc[(size_t)c * nT + t] += (uint32_t)goes_left(kind, v, thr, t);
       }
     }
@@ -645,6 +704,41 @@ int or_hist_nodes(const or_tset *ts, const or_params *pr, uint32_t nC, uint32_t 
   (void)fd; (void)fsz;
   free(slot);
   return 0;
+}
+
+/* One routing pass of a node-id image (predict.cl:36-85 with the hook's
+ * context = the tree and the image before the pass): routed pixels of a
+ * split node move to a child, the others keep their value. */
+static void ac_pass(int kind, const or_tree *tr, uint32_t nC, const or_image *im, const int32_t *in, int32_t *out)
+{
+  int fd = feat_dim_of(kind);
+  size_t fsz = feat_size(kind);
+  or_ctx cx = {tr->left, tr->post, nC, in};
+  for (int y = 0; y < im->h; y++)
+    for (int x = 0; x < im->w; x++) {
+      size_t p = (size_t)y * im->w + x;
+      int32_t n = in[p];
+      out[p] = n;
+      if (n < 0 || tr->left[n] < 0) continue;
+      const void *prm = (const char *)tr->feats + (size_t)n * fd * fsz;
+      long double v = or_feature_ctx(kind, im, x, y, prm, &cx);
+      out[p] = tr->left[n] + (goes_left(kind, v, tr->thrs, (uint32_t)n) ? 0 : 1);
+    }
+}
+
+/* The node-id image after `steps` passes from the root; sel (labels or a
+ * mask, may be NULL) chooses the routed pixels. */
+static void ac_route_image(int kind, const or_tree *tr, uint32_t nC, const or_image *im, const uint8_t *sel,
+                           uint32_t steps, int32_t *map)
+{
+  size_t P = (size_t)im->w * im->h;
+  int32_t *tmp = (int32_t *)malloc(sizeof(int32_t) * (P ? P : 1));
+  for (size_t p = 0; p < P; p++) map[p] = (sel && !sel[p]) ? -1 : 0;
+  for (uint32_t s = 0; s < steps; s++) {
+    ac_pass(kind, tr, nC, im, map, tmp);
+    memcpy(map, tmp, sizeof(int32_t) * P);
+  }
+  free(tmp);
 }
 
 /* ------------------------------------------------------------------ */
@@ -680,6 +774,12 @@ int or_train_tree(const or_tset *ts, const or_params *pr, uint32_t nC, const flo
   int32_t *snode = (int32_t *)malloc(sizeof(int32_t) * (S ? S : 1));
   unsigned char *infront = (unsigned char *)calloc(N, 1);
   or_tree tr = {left, feats, thrs, post};
+  int32_t **acmaps = NULL;
+  if (kind == OR_FEAT_AC_DEPTH) {
+    acmaps = (int32_t **)calloc(ts->n_images, sizeof(int32_t *));
+    for (uint32_t i = 0; i < ts->n_images; i++)
+      acmaps[i] = (int32_t *)malloc(sizeof(int32_t) * ((size_t)ts->widths[i] * ts->heights[i] + 1));
+  }
 
   for (uint32_t d = 1; d < end_depth; d++) {
     /* frontier */
@@ -692,7 +792,14 @@ int or_train_tree(const or_tset *ts, const or_params *pr, uint32_t nC, const flo
       for (size_t i = 0; i < lvl; i++)
         if (left[start + i] == -1 && nodeTot[start + i] > pr->leafthr) front[nfront++] = (int32_t)(start + i);
     for (uint32_t q = 0; q < nfront; q++) infront[front[q]] = 1;
-    /* route every sample from the root for d-1 levels */
+    /* route every sample from the root for d-1 levels (auto-context hooks:
+     * the labelled pixels' node-id images, d-1 passes with the current tree,
+     * cl_tree_trainer_impl_traverse_ts.hpp:211-244) */
+    if (kind == OR_FEAT_AC_DEPTH)
+      for (uint32_t i = 0; i < ts->n_images; i++) {
+        or_image im = image_of(ts, i);
+        ac_route_image(kind, &tr, nC, &im, ts->labels + ts->lab_off[i], d - 1, acmaps[i]);
+      }
     size_t g = 0;
     for (uint32_t i = 0; i < ts->n_images; i++) {
       or_image im = image_of(ts, i);
@@ -701,7 +808,7 @@ int or_train_tree(const or_tset *ts, const or_params *pr, uint32_t nC, const flo
       for (uint32_t s = 0; s < ts->n_samples[i]; s++, g++) {
         uint32_t px = smp[s];
         int x = (int)(px % (uint32_t)im.w), y = (int)(px / (uint32_t)im.w);
-        int32_t n = route(kind, &tr, &im, x, y, d - 1);
+        int32_t n = (kind == OR_FEAT_AC_DEPTH) ? acmaps[i][px] : route(kind, &tr, &im, x, y, d - 1);
         if (d == 1) classTot[(size_t)n * nC + (lab[px] - 1u)]++;   /* _traverse_ts.hpp:526 */
         /* consumer gate, _traverse_ts.hpp:530-531 */
         snode[g] = (infront[n] && nodeTot[n] > pr->leafthr) ? n : -1;
@@ -711,7 +818,8 @@ int or_train_tree(const or_tset *ts, const or_params *pr, uint32_t nC, const flo
     if (nfront == 0) { free(front); continue; }
     size_t per = (size_t)nC * nT * nF;
     uint32_t *H = (uint32_t *)malloc(sizeof(uint32_t) * per * nfront);
-    or_hist_nodes(ts, pr, nC, tree_id, nfront, front, snode, H);
+    hist_nodes_ctx(ts, pr, nC, tree_id, nfront, front, snode, H, left, post,
+                   kind == OR_FEAT_AC_DEPTH ? acmaps : NULL);
 
     for (uint32_t q = 0; q < nfront; q++) {
       uint32_t n = (uint32_t)front[q];
@@ -746,6 +854,10 @@ int or_train_tree(const or_tset *ts, const or_params *pr, uint32_t nC, const flo
     }
     free(H);
     free(front);
+  }
+  if (acmaps) {
+    for (uint32_t i = 0; i < ts->n_images; i++) free(acmaps[i]);
+    free(acmaps);
   }
   free(infront);
   free(snode);
@@ -795,6 +907,10 @@ void or_predict_leaves(int kind, const void *img, int w, int h, int nch, const u
 {
   or_image im = {img, w, h, nch};
   or_tree tr = {(int32_t *)left, (void *)feats, (void *)thrs, NULL};
+  if (kind == OR_FEAT_AC_DEPTH) {
+    (void)tr;
+    return;   /* auto-context hooks: or_predict_ac */
+  }
   for (int y = 0; y < h; y++)
     for (int x = 0; x < w; x++) {
       size_t p = (size_t)y * w + x;
@@ -821,6 +937,32 @@ void or_predict_posterior(int kind, const void *img, int w, int h, int nch, cons
       }
   }
   for (size_t i = 0; i < P * nC; i++) out[i] /= (float)n_trees;
+}
+
+/* Prediction with an auto-context hook (cl_classifier_impl.hpp:356-557):
+ * per tree, depth - 1 passes of the node-id image over the unmasked pixels
+ * with the tree's arrays as context; leaves[t] = the final image; the
+ * posterior sums post[node] in tree order and divides by the tree count
+ * (masked-out pixels: leaf -1, posterior 0). */
+void or_predict_ac(int kind, const void *img, int w, int h, int nch, const uint8_t *mask, uint32_t n_trees,
+                   const int32_t *const *lefts, const void *const *feats, const void *const *thrs,
+                   const float *const *posts, const uint32_t *depths, uint32_t nC, int32_t *leaves, float *out)
+{
+  or_image im = {img, w, h, nch};
+  size_t P = (size_t)w * h;
+  int32_t *map = (int32_t *)malloc(sizeof(int32_t) * (P ? P : 1));
+  memset(out, 0, sizeof(float) * P * nC);
+  for (uint32_t t = 0; t < n_trees; t++) {
+    or_tree tr = {(int32_t *)lefts[t], (void *)feats[t], (void *)thrs[t], (float *)posts[t]};
+    ac_route_image(kind, &tr, nC, &im, mask, depths[t] - 1, map);
+    if (leaves) memcpy(leaves + t * P, map, sizeof(int32_t) * P);
+    for (size_t p = 0; p < P; p++) {
+      if (map[p] < 0) continue;
+      for (uint32_t c = 0; c < nC; c++) out[c * P + p] += posts[t][(size_t)map[p] * nC + c];
+    }
+  }
+  for (size_t i = 0; i < P * nC; i++) out[i] /= (float)n_trees;
+  free(map);
 }
 
 int or_num_threads(void)

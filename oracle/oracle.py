@@ -14,11 +14,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "_build", "liboracle.so")
 REF_PRNG = os.path.join(HERE, "_ref", "libref_prng.so")
 
-KIND_DEPTH_F32, KIND_DEPTH_U16, KIND_RGBD, KIND_DEMO_S16 = 0, 1, 2, 3
-FD = {0: 4, 1: 4, 2: 6, 3: 2}
-FDT = {0: np.float32, 1: np.float32, 2: np.float32, 3: np.int16}
-IMGT = {0: np.float32, 1: np.uint16, 2: np.uint16, 3: np.uint16}
-NCH = {0: 1, 1: 1, 2: 4, 3: 1}
+KIND_DEPTH_F32, KIND_DEPTH_U16, KIND_RGBD, KIND_DEMO_S16, KIND_AC_DEPTH = 0, 1, 2, 3, 4
+FD = {0: 4, 1: 4, 2: 6, 3: 2, 4: 4}
+FDT = {0: np.float32, 1: np.float32, 2: np.float32, 3: np.int16, 4: np.float32}
+IMGT = {0: np.float32, 1: np.uint16, 2: np.uint16, 3: np.uint16, 4: np.float32}
+NCH = {0: 1, 1: 1, 2: 4, 3: 1, 4: 1}
 
 VP = C.c_void_p
 
@@ -87,6 +87,10 @@ def lib():
         L.or_predict_posterior.argtypes = [C.c_int, VP, C.c_int, C.c_int, C.c_int, VP, C.c_uint32, VP, VP, VP,
                                            VP, VP, C.c_uint32, VP]
         L.or_num_threads.restype = C.c_int
+        L.or_feature_ac_at.restype = C.c_float
+        L.or_feature_ac_at.argtypes = [VP, C.c_int, C.c_int, C.c_int, C.c_int, VP, VP, VP, C.c_uint32, VP]
+        L.or_predict_ac.argtypes = [C.c_int, VP, C.c_int, C.c_int, C.c_int, VP, C.c_uint32, VP, VP, VP, VP, VP,
+                                    C.c_uint32, VP, VP]
         L.or_feature_generic_at.argtypes = [C.c_int, VP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, VP, VP]
         _lib = L
     return _lib
@@ -265,6 +269,34 @@ def predict_posterior(kind, image, mask, trees, n_classes):
     lib().or_predict_posterior(kind, _p(img), w, h, NCH[kind], _p(m), nt, lefts, feats, thrs, posts, _p(depths),
                                n_classes, _p(out))
     return out.reshape(n_classes, h, w)
+
+
+def feature_ac_at(image, x, y, params, left, post, map_):
+    img = np.ascontiguousarray(image, dtype=np.float32)
+    h, w = img.shape
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    lf = np.ascontiguousarray(left, dtype=np.int32)
+    po = np.ascontiguousarray(post, dtype=np.float32)
+    mp = np.ascontiguousarray(map_, dtype=np.int32)
+    return np.float32(lib().or_feature_ac_at(_p(img), w, h, x, y, _p(p), _p(lf), _p(po), po.shape[1], _p(mp)))
+
+
+def predict_ac(kind, image, mask, trees, n_classes):
+    """Auto-context prediction: (leaves [tree][h][w], posterior [c][h][w])."""
+    img = np.ascontiguousarray(image, dtype=IMGT[kind])
+    h, w = img.shape[0], img.shape[1]
+    m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
+    nt = len(trees)
+    lefts = (VP * nt)(*[_p(t.left_children) for t in trees])
+    feats = (VP * nt)(*[_p(t.features) for t in trees])
+    thrs = (VP * nt)(*[_p(t.thresholds) for t in trees])
+    posts = (VP * nt)(*[_p(t.posteriors) for t in trees])
+    depths = np.array([t.depth for t in trees], dtype=np.uint32)
+    leaves = np.zeros(nt * w * h, dtype=np.int32)
+    out = np.zeros(n_classes * w * h, dtype=np.float32)
+    lib().or_predict_ac(kind, _p(img), w, h, NCH[kind], _p(m), nt, lefts, feats, thrs, posts, _p(depths), n_classes,
+                        _p(leaves), _p(out))
+    return leaves.reshape(nt, h, w), out.reshape(n_classes, h, w)
 
 
 def num_threads():

@@ -27,6 +27,8 @@ PLUGIN_SOURCES = [os.path.join(HERE, "examples", "depth_plugin.hip")]
 GENERIC_LIBS = {os.path.join(HERE, "libpd_channel_diff_%s.so" % k): [os.path.join(HERE, "examples",
                                                                                "channel_diff_%s.hip" % k)]
                 for k in ("a", "b")}
+# an auto-context hook (reads the tree being trained and the node-id map)
+GENERIC_LIBS[os.path.join(HERE, "libpd_context_plugin.so")] = [os.path.join(HERE, "examples", "context_plugin.hip")]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HIP_FLAGS = [
