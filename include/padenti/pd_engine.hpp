@@ -332,6 +332,7 @@ struct TrainerBase {
   uint32_t nC = 0;
   uint64_t hist_budget = 8ull << 30;
   uint32_t chunk = 0;
+  bool probe_order = true;   // probe-coherent candidate tiles (functors with kProbeParam)
   pd_tuning tune{8u, 16u};   // waves per histogram block, target blocks per CU
   std::vector<pd_tuning_rule> tune_rules;   // per level, by samples per node (decreasing)
   pd_tuning level_tuning(uint64_t active, uint32_t trainable) const
@@ -402,6 +403,8 @@ struct Trainer final : TrainerBase {
   size_t lds_limit = 65536;
   // candidate map of the histogram launch in flight (proposal slices)
   const uint32_t *d_map_cur = nullptr;
+  DevBuf d_perm;
+  const uint16_t *d_perm_cur = nullptr;   // [slice slot][nF] candidate order of the current launch
   DevBuf d_gains, d_emit_slots, d_emit_base, d_emit_best, d_emit_in, d_emit_node, d_pslot_in, d_pslot_node, d_scores,
       d_map;
 
@@ -566,7 +569,7 @@ struct Trainer final : TrainerBase {
                        d_imgs.as<ImgDesc>(), act, d_items.as<HistItem>(), item_base, d_slot_node.as<int32_t>(),
                        d_slot_mode.as<uint8_t>(), d_slot_hidx.as<int32_t>(), d_tot.as<uint32_t>(), bnd, tree, nF,
                        nT, nC, d_H.as<uint32_t>(), d_cand_key.as<unsigned long long>(), d_cand_left.as<uint32_t>(),
-                       d_map_cur, ac);
+                       d_map_cur, d_perm_cur, ac);
     HIPCHK(hipGetLastError());
   }
 
@@ -663,6 +666,16 @@ struct Trainer final : TrainerBase {
       HIPCHK(hipMemcpyAsync(d_zero.p, zero.data(), zero.size() * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
     }
     if (ev_begin) HIPCHK(hipEventRecord(ev_begin->e, stream));
+    d_perm_cur = nullptr;
+    if constexpr (probe_param<F>::value >= 0) {
+      if (probe_order && nF > 64 && nF <= ORDER_MAX) {
+        d_perm.ensure((size_t)ns * nF * sizeof(uint16_t));
+        hipLaunchKernelGGL((k_cand_order<F>), dim3(ns), dim3(ORDER_THREADS), 0, stream, tree,
+                           d_slot_node.as<int32_t>(), nF, bnd, d_map_cur, d_perm.as<uint16_t>());
+        HIPCHK(hipGetLastError());
+        d_perm_cur = d_perm.as<uint16_t>();
+      }
+    }
     if (!zero.empty()) {
       const size_t total = zero.size() * per;
       const uint32_t blocks = (uint32_t)std::min<size_t>((total + 255) / 256, (size_t)cu_count * 64);
@@ -682,6 +695,7 @@ struct Trainer final : TrainerBase {
         launches++;
       }
     }
+    d_perm_cur = nullptr;
     if (ev_end) HIPCHK(hipEventRecord(ev_end->e, stream));
     return launches;
   }

@@ -94,6 +94,34 @@ PD_HD float div_by(float p, const DivPrep &dp, bool fast)
 }
 PD_HD bool div_fast_ok(const DivPrep &dp) { return dp.d >= 0x1p-16f && dp.d <= 0x1p16f; }
 
+// Denominator that is an integer with 1 <= |d| <= 65535 (a uint16 or int16
+// depth): d and its correctly rounded reciprocal y = RN(1/d) (an IEEE
+// division, once per pixel).  Then one FMA residual correction of p * y is
+// already RN(p/d) for every float p whose quotient is a normal number:
+// with q0 = RN(p*y), |q0 - p/d| < 2^(e-22) for |p/d| in [2^e, 2^(e+1)), and
+// z = q0 + RN(p - d*q0)*y (the residual is exact) satisfies
+// z - p/d = (q0 - p/d)(1 - d*y), so |z - p/d| < 2^(e-46); every rounding
+// midpoint m near p/d is an odd multiple of 2^(e-25) or 2^(e-24) with a
+// 25-bit numerator, so p - m*d is a non-zero multiple of 2^(e-25) and
+// |p/d - m| >= 2^(e-25)/|d| > 2^(e-41): no midpoint lies between z and p/d.
+// (Subnormal quotients are below 0.5 and round to offset 0 either way.)
+struct IntDivPrep {
+  float d, y;
+};
+
+PD_HD IntDivPrep int_div_prep(float d) { return IntDivPrep{d, 1.0f / d}; }
+
+PD_HD float int_div(float p, const IntDivPrep &dp)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+  const float q = p * dp.y;
+  const float r = __builtin_fmaf(-dp.d, q, p);
+  return __builtin_fmaf(r, dp.y, q);
+#else
+  return p / dp.d;
+#endif
+}
+
 // Integer offset round(q) (half away from zero) of a quotient, with every
 // quotient whose magnitude is not below 2^20 (or NaN) mapped to +-2^20, an
 // offset that leaves any image (width, height < 2^16): the same pixel/border
@@ -155,25 +183,38 @@ struct DepthF32Feature {
   }
 };
 
+// probe_div with an integer depth (IntDivPrep).  Only the load is
+// predicated: lanes whose probe leaves the image issue no memory access.
+template <typename ImgT, int NCH>
+PD_HD float probe_int(const ImageView<ImgT, NCH> &im, int x, int y, const IntDivPrep &dp, float px, float py, int ch,
+                      float bg)
+{
+  const int ox = x + offset_of(int_div(px, dp));
+  const int oy = y + offset_of(int_div(py, dp));
+  float v = bg;
+  if ((uint32_t)ox < (uint32_t)im.w && (uint32_t)oy < (uint32_t)im.h) v = (float)im.at(ox, oy, ch);
+  return v;
+}
+
 // C2/C3: uint16 depth in millimetres, float response; p in pixel*mm,
 // out-of-image reads 10000 mm, d(x) == 0 -> 0.
 struct DepthU16Feature {
   using ImgType = uint16_t;
   using FeatType = float;
-  using Prep = DivPrep;
+  using Prep = IntDivPrep;
   static constexpr int nChannels = 1;
   static constexpr int FeatDim = 4;
   static constexpr int kind = 1;
+  static constexpr int kProbeParam = 0;   // p[0], p[1]: offset of the first probe
   PD_HD static Prep prepare(const ImageView<uint16_t, 1> &im, int x, int y)
   {
-    return div_prep((float)im.at(x, y, 0));
+    return int_div_prep((float)im.at(x, y, 0));
   }
   PD_HD static float eval_prep(const ImageView<uint16_t, 1> &im, int x, int y, Prep dp, const float *p)
   {
     if (!(dp.d > 0.0f)) return 0.0f;
-    const bool fast = div_fast_ok(dp);
-    float a = probe_div(im, x, y, dp, fast, p[0], p[1], 0, 10000.0f);
-    float b = probe_div(im, x, y, dp, fast, p[2], p[3], 0, 10000.0f);
+    const float a = probe_int(im, x, y, dp, p[0], p[1], 0, 10000.0f);
+    const float b = probe_int(im, x, y, dp, p[2], p[3], 0, 10000.0f);
     return a - b;
   }
   PD_HD static float eval(const ImageView<uint16_t, 1> &im, int x, int y, const float *p)
@@ -189,10 +230,11 @@ struct DepthU16Feature {
 struct RGBDFeature {
   using ImgType = uint16_t;
   using FeatType = float;
-  using Prep = DivPrep;
+  using Prep = IntDivPrep;
   static constexpr int nChannels = 4;
   static constexpr int FeatDim = 6;
   static constexpr int kind = 2;
+  static constexpr int kProbeParam = 0;
   PD_HD static int chan(float c)
   {
     if (!(c > 0.0f)) return 0;
@@ -201,14 +243,13 @@ struct RGBDFeature {
   }
   PD_HD static Prep prepare(const ImageView<uint16_t, 4> &im, int x, int y)
   {
-    return div_prep((float)im.at(x, y, 3));
+    return int_div_prep((float)im.at(x, y, 3));
   }
   PD_HD static float eval_prep(const ImageView<uint16_t, 4> &im, int x, int y, Prep dp, const float *p)
   {
     if (!(dp.d > 0.0f)) return 0.0f;
-    const bool fast = div_fast_ok(dp);
-    float a = probe_div(im, x, y, dp, fast, p[0], p[1], chan(p[4]), 10000.0f);
-    float b = probe_div(im, x, y, dp, fast, p[2], p[3], chan(p[5]), 10000.0f);
+    const float a = probe_int(im, x, y, dp, p[0], p[1], chan(p[4]), 10000.0f);
+    const float b = probe_int(im, x, y, dp, p[2], p[3], chan(p[5]), 10000.0f);
     return a - b;
   }
   PD_HD static float eval(const ImageView<uint16_t, 4> &im, int x, int y, const float *p)
@@ -224,22 +265,21 @@ struct RGBDFeature {
 struct DemoS16Feature {
   using ImgType = uint16_t;
   using FeatType = int16_t;
-  using Prep = DivPrep;   // d = (float)(short)I(x), exact
+  using Prep = IntDivPrep;   // d = (float)(short)I(x), exact, |d| <= 32768
   static constexpr int nChannels = 1;
   static constexpr int FeatDim = 2;
   static constexpr int kind = 3;
+  static constexpr int kProbeParam = 0;
   PD_HD static Prep prepare(const ImageView<uint16_t, 1> &im, int x, int y)
   {
-    return div_prep((float)(int16_t)im.at(x, y, 0));
+    return int_div_prep((float)(int16_t)im.at(x, y, 0));
   }
   PD_HD static int16_t eval_prep(const ImageView<uint16_t, 1> &im, int x, int y, Prep dp, const int16_t *p)
   {
     const int16_t depth = (int16_t)dp.d;
     if (depth == 0) return 0;
-    // a negative depth (a wrapped uint16) takes the plain division
-    const bool fast = div_fast_ok(dp);
-    const int ox = x + offset_of(div_by((float)p[0] * 500.0f, dp, fast));
-    const int oy = y + offset_of(div_by((float)p[1] * 500.0f, dp, fast));
+    const int ox = x + offset_of(int_div((float)p[0] * 500.0f, dp));
+    const int oy = y + offset_of(int_div((float)p[1] * 500.0f, dp));
     int16_t d2 = (int16_t)10000;
     if ((uint32_t)ox < (uint32_t)im.w && (uint32_t)oy < (uint32_t)im.h) d2 = (int16_t)im.at(ox, oy, 0);
     return (int16_t)(depth - (d2 ? d2 : (int16_t)10000));

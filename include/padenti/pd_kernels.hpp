@@ -113,23 +113,29 @@ __global__ void k_prepare(const typename F::ImgType *__restrict__ pixels, const 
   recs[i] = r;
 }
 
-// One (sample, candidate) pair: response, rank r = #{t : !(v <= thr_t)} by a
-// branch-free binary search over the lane's sorted thresholds (the
-// predicate is true exactly on a prefix of the sorted list; a NaN response
-// makes it true everywhere), then one LDS add into bin [class][r].
+// Response of one (sample, candidate) pair.
+template <class F>
+__device__ __forceinline__ typename F::FeatType hist_eval(const typename F::ImgType *__restrict__ pixels,
+                                                          const ImgDesc *__restrict__ imgs, uint4 r,
+                                                          const typename F::FeatType (&p)[F::FeatDim],
+                                                          const AcArgs &ac)
+{
+  const ImgDesc d = imgs[rec_img(r)];
+  ImageView<typename F::ImgType, F::nChannels> view{pixels + d.off, (int)d.w, (int)d.h};
+  return eval_hook<F>(view, rec_x(r), rec_y(r), unpack_prep<typename F::Prep>(r.z, r.w), p, ac, rec_img(r));
+}
+
+// Rank r = #{t : !(v <= thr_t)} of response v by a branch-free binary search
+// over the lane's sorted thresholds (the predicate is true exactly on a
+// prefix of the sorted list; a NaN response makes it true everywhere), then
+// one LDS add into bin [class][r].
 template <class F, int NTP>
-__device__ __forceinline__ void hist_sample(const typename F::ImgType *__restrict__ pixels,
-                                            const ImgDesc *__restrict__ imgs, uint4 r,
-                                            const typename F::FeatType (&p)[F::FeatDim],
-                                            const typename F::FeatType *sth, uint32_t nT, bool fv,
-                                            uint32_t lane, uint32_t *bins, const AcArgs &ac)
+__device__ __forceinline__ void hist_add(typename F::FeatType v, uint4 r, const typename F::FeatType *sth,
+                                         uint32_t nT, bool fv, uint32_t lane, uint32_t *bins)
 {
   using T = typename F::FeatType;
   constexpr int RW = rank_words<NTP>();
   constexpr int NS = sorted_slots<NTP>();
-  const ImgDesc d = imgs[rec_img(r)];
-  ImageView<typename F::ImgType, F::nChannels> view{pixels + d.off, (int)d.w, (int)d.h};
-  const T v = eval_hook<F>(view, rec_x(r), rec_y(r), unpack_prep<typename F::Prep>(r.z, r.w), p, ac, rec_img(r));
   // binary search carried as the byte offset of slot `pos` in the lane's
   // column (slot stride 64 * sizeof(T)); the step offsets fold into the LDS
   // read's immediate
@@ -163,6 +169,87 @@ __device__ __forceinline__ uint32_t prefix_at(const uint32_t *bins, int RW, uint
   return (bins[(c * RW + (l >> 1)) * 64u + lane] >> ((l & 1u) * 16u)) & 0xffffu;
 }
 
+// ---------------------------------------------------------------------------
+// Probe-coherent candidate order.  A functor may declare
+//   static constexpr int kProbeParam = j;
+// when p[j], p[j+1] are the x / y offset numerators of its first texel probe.
+// The trainer then lays each node's candidates out along a Morton curve of
+// (p[j], p[j+1]) before cutting them into 64-lane tiles: the first probes of
+// the lanes of one gather land close together and touch far fewer cache
+// lines.  The order only decides which lane evaluates which candidate; every
+// count, key and output column stays that of the candidate's own index.
+// ---------------------------------------------------------------------------
+template <class F, class = void>
+struct probe_param : std::integral_constant<int, -1> {};
+template <class F>
+struct probe_param<F, std::void_t<decltype(F::kProbeParam)>> : std::integral_constant<int, F::kProbeParam> {};
+
+constexpr uint32_t ORDER_MAX = 4096;   // candidates per node the order kernel sorts in LDS
+constexpr uint32_t ORDER_THREADS = 256;
+
+// 10-bit grid coordinate of v in [lo, hi] spread to the even bits
+template <typename T>
+__device__ __forceinline__ uint32_t order_coord(T v, T lo, T hi)
+{
+  const float span = (float)hi - (float)lo;
+  float t = span > 0.0f ? ((float)v - (float)lo) / span : 0.0f;
+  t = fminf(fmaxf(t, 0.0f), 1.0f);
+  uint32_t q = (uint32_t)(t * 1023.0f);
+  q = (q | (q << 8)) & 0x00ff00ffu;
+  q = (q | (q << 4)) & 0x0f0f0f0fu;
+  q = (q | (q << 2)) & 0x33333333u;
+  q = (q | (q << 1)) & 0x55555555u;
+  return q;
+}
+
+// One block per slot: perm[q][j] = the candidate (output column) at tile
+// position j of slot q, sorted by (Morton key of the first probe's offset,
+// column): a bitonic sort of nF <= ORDER_MAX 64-bit keys in LDS.
+template <class F>
+__global__ __launch_bounds__(ORDER_THREADS) void k_cand_order(uint32_t tree, const int32_t *__restrict__ slot_node,
+                                                              uint32_t nF, Bounds<F> bnd,
+                                                              const uint32_t *__restrict__ cand_map,
+                                                              uint16_t *__restrict__ perm)
+{
+  using T = typename F::FeatType;
+  constexpr int J = probe_param<F>::value;
+  static_assert(J >= 0 && J + 1 < F::FeatDim, "kProbeParam names two parameters");
+  __shared__ unsigned long long key[ORDER_MAX];
+  const uint32_t q = blockIdx.x;
+  const uint32_t node = (uint32_t)slot_node[q];
+  uint32_t n2 = 1;
+  while (n2 < nF) n2 <<= 1;
+  for (uint32_t f = threadIdx.x; f < n2; f += ORDER_THREADS) {
+    unsigned long long k = ~0ull;   // padding sorts last
+    if (f < nF) {
+      const uint32_t fs = cand_map ? cand_map[(size_t)q * nF + f] : f;
+      T p[F::FeatDim];
+      candidate_params<T, F::FeatDim>(tree, node, fs, bnd.lo, bnd.hi, p);
+      const uint32_t m = order_coord<T>(p[J], bnd.lo[J], bnd.hi[J]) |
+                         (order_coord<T>(p[J + 1], bnd.lo[J + 1], bnd.hi[J + 1]) << 1);
+      k = ((unsigned long long)m << 32) | f;
+    }
+    key[f] = k;
+  }
+  __syncthreads();
+  for (uint32_t size = 2; size <= n2; size <<= 1) {
+    for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+      for (uint32_t i = threadIdx.x; i < n2 / 2; i += ORDER_THREADS) {
+        const uint32_t a = 2u * i - (i & (stride - 1u));   // partner a + stride, bit `stride` of a clear
+        const unsigned long long x = key[a], y = key[a + stride];
+        const bool up = (a & size) == 0u;
+        if ((x > y) == up) {
+          key[a] = y;
+          key[a + stride] = x;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (uint32_t j = threadIdx.x; j < nF; j += ORDER_THREADS)
+    perm[(size_t)q * nF + j] = (uint16_t)(key[j] & 0xffffu);
+}
+
 template <class F, int NTP, int NWAVES = HIST_THREADS / 64>
 __global__ __launch_bounds__(NWAVES * 64) void k_hist(
     const typename F::ImgType *__restrict__ pixels, const ImgDesc *__restrict__ imgs,
@@ -170,7 +257,8 @@ __global__ __launch_bounds__(NWAVES * 64) void k_hist(
     const int32_t *__restrict__ slot_node, const uint8_t *__restrict__ slot_mode,
     const int32_t *__restrict__ slot_hidx, const uint32_t *__restrict__ tot, Bounds<F> bnd, uint32_t tree,
     uint32_t nF, uint32_t nT, uint32_t nC, uint32_t *__restrict__ H, unsigned long long *__restrict__ cand_key,
-    uint32_t *__restrict__ cand_left, const uint32_t *__restrict__ cand_map, AcArgs ac)
+    uint32_t *__restrict__ cand_left, const uint32_t *__restrict__ cand_map, const uint16_t *__restrict__ perm,
+    AcArgs ac)
 {
   using T = typename F::FeatType;
   constexpr int FD = F::FeatDim;
@@ -191,8 +279,11 @@ __global__ __launch_bounds__(NWAVES * 64) void k_hist(
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = tid & 63u;
   const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const uint32_t f = it.ftile * 64u + lane;
-  const bool fv = f < nF;
+  // tile position tp; the lane's candidate (output column) f is tp itself or
+  // the slot's probe-coherent order at tp
+  const uint32_t tp = it.ftile * 64u + lane;
+  const bool fv = tp < nF;
+  const uint32_t f = (perm && fv) ? (uint32_t)perm[(size_t)it.slot * nF + tp] : tp;
   const uint32_t node = (uint32_t)slot_node[it.slot];
   // sampler index of the lane's candidate: f itself, or (proposal) the
   // f-th entry of the slot's selected candidate list
@@ -241,20 +332,25 @@ __global__ __launch_bounds__(NWAVES * 64) void k_hist(
   for (uint32_t i = tid; i < nC * RW * 64u; i += NTH) bins[i] = 0u;
   __syncthreads();
 
-  // Sample loop, two samples in flight per wave.  r(v) = #{t : !(v <= thr_t)}
+  // Sample loop, two samples per wave and step.  r(v) = #{t : !(v <= thr_t)}
   // is #{t : thr_t < v} for a finite v and >= nT for NaN (padded slots never
   // count for finite v).  v <= thr_t  <=>  r(v) <= L_t:  if v <= thr_t every
   // thr below v is below thr_t; if v > thr_t then thr_t and every thr <= thr_t
-  // are below v, so r(v) >= L_t + 1.
+  // are below v, so r(v) >= L_t + 1.  (A variant with unpredicated probes
+  // that issued both samples' gathers before either bin update measured
+  // slower: the gathers are bound by throughput, not latency.)
   const uint32_t end = it.begin + it.count;
   uint32_t i = it.begin + wave;
   for (; i + NW < end; i += 2 * NW) {
     const uint4 ra = recs[i];
     const uint4 rb = recs[i + NW];
-    hist_sample<F, NTP>(pixels, imgs, ra, p, sth, nT, fv, lane, bins, ac);
-    hist_sample<F, NTP>(pixels, imgs, rb, p, sth, nT, fv, lane, bins, ac);
+    hist_add<F, NTP>(hist_eval<F>(pixels, imgs, ra, p, ac), ra, sth, nT, fv, lane, bins);
+    hist_add<F, NTP>(hist_eval<F>(pixels, imgs, rb, p, ac), rb, sth, nT, fv, lane, bins);
   }
-  if (i < end) hist_sample<F, NTP>(pixels, imgs, recs[i], p, sth, nT, fv, lane, bins, ac);
+  if (i < end) {
+    const uint4 ra = recs[i];
+    hist_add<F, NTP>(hist_eval<F>(pixels, imgs, ra, p, ac), ra, sth, nT, fv, lane, bins);
+  }
   __syncthreads();
 
   // Flush: per class (classes strided over waves) turn this lane's rank bins
@@ -331,9 +427,11 @@ __global__ __launch_bounds__(NWAVES * 64) void k_hist(
   const uint32_t ftiles = (nF + 63) / 64;
   const size_t ci = (size_t)it.slot * ftiles + it.ftile;
   if (tid == 0) cand_key[ci] = m;
+  // lane of the winning column (every wave holds the same lane -> column map)
+  const uint64_t wmask = __ballot(fv && f == key_index(m) % nF);
   if (m != 0ull && tid < nC) {
     const uint32_t k = key_index(m);
-    const uint32_t wl = (k % nF) - it.ftile * 64u;
+    const uint32_t wl = (uint32_t)__builtin_ctzll(wmask);
     const uint32_t l = sl[(k / nF) * 64 + wl];
     cand_left[ci * nC + tid] = prefix_at(bins, RW, tid, l, wl);
   }
@@ -874,6 +972,13 @@ __global__ void k_div(const float *__restrict__ p, const float *__restrict__ d, 
   if (i >= n) return;
   const DivPrep dp = div_prep(d[i]);
   q[i] = div_by(p[i], dp, div_fast_ok(dp));
+}
+
+__global__ void k_div_int(const float *__restrict__ p, const float *__restrict__ d, float *__restrict__ q, uint32_t n)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  q[i] = int_div(p[i], int_div_prep(d[i]));
 }
 
 // Plain streaming copy (16 B per lane per step): HBM bandwidth probe and the

@@ -138,6 +138,13 @@ PD_API int pd_trainer_set_data(pd_trainer *tr, uint32_t n_images, const void *pi
 PD_API int pd_trainer_set_hist_budget(pd_trainer *tr, uint64_t bytes);
 /* Samples per histogram work item (0 = automatic). */
 PD_API int pd_trainer_set_chunk(pd_trainer *tr, uint32_t samples_per_chunk);
+/* Probe-coherent candidate tiles (default on): for feature hooks that name
+ * their first probe's offset parameters, each node's candidates are handed
+ * to the 64-lane histogram tiles in Morton order of that offset, so one
+ * texel gather touches fewer cache lines.  Results are identical either
+ * way.  No counterpart in the reference (hist_update.cl assigns candidate
+ * f to work item f). */
+PD_API int pd_trainer_set_probe_order(pd_trainer *tr, int enable);
 
 /* Train one tree: TreeTrainer::train(tree, ts, params, 1, end_depth)
  * (tree_trainer.hpp:82-85).  Tree arrays are the reference SoA layout of a
@@ -190,6 +197,9 @@ PD_API int pd_md5rand(int device, uint32_t n, const uint32_t *seeds, uint32_t *o
 PD_API int pd_log2_det(int device, uint32_t n, const float *x, float *y);
 /* q[i] = p[i] / d[i] through the feature hooks' per-pixel division path. */
 PD_API int pd_fast_div(int device, uint32_t n, const float *p, const float *d, float *q);
+/* q[i] = p[i] / d[i] through the integer-depth hooks' division (d[i] an
+ * integer, 1 <= |d[i]| <= 65535). */
+PD_API int pd_int_div(int device, uint32_t n, const float *p, const float *d, float *q);
 PD_API int pd_feature_eval(int device, int feature_kind, const void *image, uint32_t w, uint32_t h, uint32_t n,
                     const int32_t *xy, const void *params, void *out);
 

@@ -206,6 +206,10 @@ class HIPTreeTrainer:
     def set_chunk(self, samples_per_chunk):
         check(lib().pd_trainer_set_chunk(self._h, int(samples_per_chunk)))
 
+    def set_probe_order(self, enable):
+        """Probe-coherent candidate tiles on/off (results are identical)."""
+        check(lib().pd_trainer_set_probe_order(self._h, 1 if enable else 0))
+
     def set_record_sink(self, buf, per_node):
         """Plain trainings emit per_node records per trainable node into the
         RecordBuffer `buf` (appending after its current count); None removes it."""

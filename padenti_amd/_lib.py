@@ -140,6 +140,7 @@ def lib():
         "pd_trainer_set_data": (I32, [VP, U32, VP, VP, VP, VP, VP, VP, VP, VP, VP]),
         "pd_trainer_set_hist_budget": (I32, [VP, U64]),
         "pd_trainer_set_chunk": (I32, [VP, U32]),
+        "pd_trainer_set_probe_order": (I32, [VP, I32]),
         "pd_trainer_set_tuning": (I32, [VP, C.POINTER(Tuning)]),
         "pd_trainer_get_tuning": (I32, [VP, C.POINTER(Tuning)]),
         "pd_trainer_set_tuning_rules": (I32, [VP, U32, VP]),
@@ -152,6 +153,7 @@ def lib():
         "pd_md5rand": (I32, [I32, U32, VP, VP]),
         "pd_log2_det": (I32, [I32, U32, VP, VP]),
         "pd_fast_div": (I32, [I32, U32, VP, VP, VP]),
+        "pd_int_div": (I32, [I32, U32, VP, VP, VP]),
         "pd_feature_eval": (I32, [I32, I32, VP, U32, U32, U32, VP, VP, VP]),
         "pd_forest_create": (I32, [C.POINTER(VP), I32, I32, U32]),
         "pd_forest_destroy": (I32, [VP]),

@@ -65,6 +65,14 @@ int pd_trainer_set_chunk(pd_trainer *tr, uint32_t samples_per_chunk)
   });
 }
 
+int pd_trainer_set_probe_order(pd_trainer *tr, int enable)
+{
+  return guarded([&] {
+    require(tr != nullptr, "null trainer");
+    tr->impl->probe_order = enable != 0;
+  });
+}
+
 int pd_trainer_set_tuning(pd_trainer *tr, const pd_tuning *tuning)
 {
   return guarded([&] {
@@ -213,6 +221,24 @@ int pd_fast_div(int device, uint32_t n, const float *p, const float *d, float *q
     HIPCHK(hipMemcpy(dp.p, p, (size_t)n * 4, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(dd.p, d, (size_t)n * 4, hipMemcpyHostToDevice));
     hipLaunchKernelGGL(k_div, dim3((n + 255) / 256), dim3(256), 0, nullptr, dp.as<float>(), dd.as<float>(),
+                       dq.as<float>(), n);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpy(q, dq.p, (size_t)n * 4, hipMemcpyDeviceToHost));
+  });
+}
+
+int pd_int_div(int device, uint32_t n, const float *p, const float *d, float *q)
+{
+  return guarded([&] {
+    require(n >= 1 && p && d && q, "bad division arguments");
+    set_device(device);
+    DevBuf dp, dd, dq;
+    dp.ensure((size_t)n * 4);
+    dd.ensure((size_t)n * 4);
+    dq.ensure((size_t)n * 4);
+    HIPCHK(hipMemcpy(dp.p, p, (size_t)n * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dd.p, d, (size_t)n * 4, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_div_int, dim3((n + 255) / 256), dim3(256), 0, nullptr, dp.as<float>(), dd.as<float>(),
                        dq.as<float>(), n);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpy(q, dq.p, (size_t)n * 4, hipMemcpyDeviceToHost));

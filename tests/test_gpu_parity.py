@@ -62,6 +62,50 @@ def test_fast_division_device(gpu):
     assert np.array_equal(q2[:4], (p2 / d2)[:4].astype(np.float32))
 
 
+def _midpoint_numerators(d, rng):
+    """Numerators p whose quotient p/d lies next to a rounding midpoint of
+    float32 (the cases a one-step quotient could round the wrong way): for
+    a random quotient q, the midpoint m between q and its successor, and
+    p = RN(m * d) and its float neighbours."""
+    q = rng.uniform(0.5, 200.0, d.size).astype(np.float32) * rng.choice([-1, 1], d.size).astype(np.float32)
+    q = q.astype(np.float32)
+    m = (q.astype(np.float64) + np.nextafter(q, np.float32(np.inf) * np.sign(q)).astype(np.float64)) / 2
+    p0 = (m * d.astype(np.float64)).astype(np.float32)
+    return [p0, np.nextafter(p0, np.float32(np.inf)), np.nextafter(p0, np.float32(-np.inf))]
+
+
+def test_int_division_device(gpu):
+    """The integer-depth hooks' one-correction quotient equals IEEE float32
+    division for every integer depth 1..65535 and -32768..-1: random
+    numerators over the offset range, integral ones, half-integer
+    quotients, and numerators next to rounding midpoints."""
+    rng = np.random.default_rng(12)
+    dd = np.concatenate([np.arange(1, 65536), -np.arange(1, 32769)]).astype(np.float32)
+    ps, ds = [], []
+    for _ in range(3):
+        ps.append(rng.uniform(-2.0e5, 2.0e5, dd.size).astype(np.float32))
+        ds.append(dd)
+    ps.append(rng.integers(-200000, 200000, dd.size).astype(np.float32))
+    ds.append(dd)
+    ps.append(((rng.integers(-300, 300, dd.size) + 0.5) * dd).astype(np.float32))   # p/d = k + 0.5
+    ds.append(dd)
+    for pm in _midpoint_numerators(dd, rng):
+        ps.append(pm)
+        ds.append(dd)
+    mag = np.exp2(rng.uniform(-60, 60, dd.size)).astype(np.float32)
+    ps.append((mag * rng.choice([-1.0, 1.0], dd.size)).astype(np.float32))
+    ds.append(dd)
+    p = np.concatenate(ps)
+    d = np.concatenate(ds)
+    q = np.zeros_like(p)
+    L.check(L.lib().pd_int_div(0, p.size, L.ptr(p), L.ptr(d), L.ptr(q)))
+    want = (p / d).astype(np.float32)
+    normal = np.abs(want) >= np.float32(2.0 ** -126)
+    assert np.array_equal(q[normal], want[normal])
+    # subnormal quotients are below 0.5 and only ever round to offset 0
+    assert np.all(np.abs(q[~normal]) < 0.5)
+
+
 def _rand_image(kind, rng, w, h):
     if kind == pa.PD_FEAT_DEPTH_F32:
         im = rng.uniform(0.5, 6.0, (h, w)).astype(np.float32)
@@ -159,6 +203,61 @@ def test_hist_nodes_device(gpu, kind, nF, nT, chunk):
     want = orc.hist_nodes(orc.Tset(kind, ts), p, 2, nodes, sample_node)
     assert np.array_equal(H, want)
     tr.close()
+
+
+@pytest.mark.parametrize("kind,nF,nT", [
+    (pa.PD_FEAT_DEPTH_U16, 200, 20),      # 4 tiles, the last one partial
+    (pa.PD_FEAT_DEMO_S16, 129, 6),        # the single-probe demo hook, 1-lane last tile
+    (pa.PD_FEAT_RGBD_U16X4, 300, 9),
+])
+def test_probe_order_histograms_identical(gpu, kind, nF, nT):
+    """Probe-coherent candidate tiles change which lane evaluates which
+    candidate, never a count: hist_nodes with the order on and off both
+    equal the oracle, in fused (single-chunk) and chunked nodes."""
+    rng = np.random.default_rng(300 + nF)
+    ts = _tiny_ts(kind, rng)
+    fd = L.FEATURES[kind][3]
+    if kind == pa.PD_FEAT_DEMO_S16:
+        lo, hi, tb = [-60] * fd, [60] * fd, (-300, 300)
+    else:
+        lo, hi, tb = [-100000.0] * fd, [100000.0] * fd, (-800.0, 800.0)
+        if kind == pa.PD_FEAT_RGBD_U16X4:
+            lo[4:], hi[4:] = [0.0, 0.0], [4.0, 4.0]
+    p = pa.TreeTrainerParameters(nF, nT, lo, hi, tb[0], tb[1], 0)
+    S = sum(im.samples.size for im in ts.images)
+    nodes = np.array([0, 5, 6], dtype=np.int32)
+    sample_node = rng.choice([0, 5, 6, -1], S).astype(np.int32)
+    want = orc.hist_nodes(orc.Tset(kind, ts), p, 1, nodes, sample_node)
+    for order in (True, False):
+        for chunk in (0, 50):
+            tr = pa.HIPTreeTrainer(kind, ts.n_classes)
+            tr.set_training_set(ts)
+            tr.set_probe_order(order)
+            if chunk:
+                tr.set_chunk(chunk)
+            H = tr.hist_nodes(p, 1, nodes, sample_node)
+            tr.close()
+            assert np.array_equal(H, want), (order, chunk)
+
+
+def test_probe_order_tree_identical(gpu):
+    """A C2-shaped tree with fused in-kernel split search (the winning lane
+    is found through the candidate order) is the same with the order off."""
+    ts = synth.training_set("c2", n_images=6, samples=300)
+    p = synth.params_for("c2", n_features=500)
+    trees = []
+    for order in (True, False):
+        tree = pa.Tree(0, 8, pa.PD_FEAT_DEPTH_U16, 8)
+        tr = pa.HIPTreeTrainer(pa.PD_FEAT_DEPTH_U16, 8)
+        tr.set_training_set(ts)
+        tr.set_probe_order(order)
+        tr.train(tree, ts, p, 1, 8)
+        tr.close()
+        trees.append(tree)
+    a, b = trees
+    assert np.array_equal(a.left_children, b.left_children)
+    assert np.array_equal(a.features, b.features) and np.array_equal(a.thresholds, b.thresholds)
+    assert np.array_equal(a.posteriors, b.posteriors)
 
 
 def test_split_search_device(gpu):
