@@ -405,6 +405,22 @@ struct Trainer final : TrainerBase {
   const uint32_t *d_map_cur = nullptr;
   DevBuf d_perm;
   const uint16_t *d_perm_cur = nullptr;   // [slice slot][nF] candidate order of the current launch
+  PinnedStage stage;   // per-level transfers; reset after every stream synchronization
+
+  void h2d(void *dst, const void *src, size_t bytes)
+  {
+    if (!bytes) return;
+    HIPCHK(hipMemcpyAsync(dst, stage.put(src, bytes), bytes, hipMemcpyHostToDevice, stream));
+  }
+  // device -> host, synchronizing the stream (which also retires the staging)
+  void d2h_sync(void *dst, const void *src, size_t bytes)
+  {
+    void *pin = bytes ? stage.alloc(bytes) : nullptr;
+    if (bytes) HIPCHK(hipMemcpyAsync(pin, src, bytes, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    if (bytes) std::copy((const char *)pin, (const char *)pin + bytes, (char *)dst);
+    stage.reset();
+  }
   DevBuf d_gains, d_emit_slots, d_emit_base, d_emit_best, d_emit_in, d_emit_node, d_pslot_in, d_pslot_node, d_scores,
       d_map;
 
@@ -656,14 +672,14 @@ struct Trainer final : TrainerBase {
     d_H.ensure(per * std::max<size_t>(hslots.size(), 1) * sizeof(uint32_t));
     d_cand_key.ensure((size_t)ns * ftiles * sizeof(unsigned long long));
     d_cand_left.ensure((size_t)ns * ftiles * nC * sizeof(uint32_t));
-    HIPCHK(hipMemcpyAsync(d_items.p, items.data(), items.size() * sizeof(HistItem), hipMemcpyHostToDevice, stream));
-    HIPCHK(hipMemcpyAsync(d_slot_node.p, snode.data(), ns * sizeof(int32_t), hipMemcpyHostToDevice, stream));
-    HIPCHK(hipMemcpyAsync(d_slot_mode.p, smode.data(), ns, hipMemcpyHostToDevice, stream));
-    HIPCHK(hipMemcpyAsync(d_slot_hidx.p, shidx.data(), ns * sizeof(int32_t), hipMemcpyHostToDevice, stream));
-    HIPCHK(hipMemcpyAsync(d_tot.p, tot.data(), (size_t)ns * nC * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+    h2d(d_items.p, items.data(), items.size() * sizeof(HistItem));
+    h2d(d_slot_node.p, snode.data(), ns * sizeof(int32_t));
+    h2d(d_slot_mode.p, smode.data(), ns);
+    h2d(d_slot_hidx.p, shidx.data(), ns * sizeof(int32_t));
+    h2d(d_tot.p, tot.data(), (size_t)ns * nC * sizeof(uint32_t));
     if (!zero.empty()) {
       d_zero.ensure(zero.size() * sizeof(uint32_t));
-      HIPCHK(hipMemcpyAsync(d_zero.p, zero.data(), zero.size() * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+      h2d(d_zero.p, zero.data(), zero.size() * sizeof(uint32_t));
     }
     if (ev_begin) HIPCHK(hipEventRecord(ev_begin->e, stream));
     d_perm_cur = nullptr;
@@ -735,7 +751,7 @@ struct Trainer final : TrainerBase {
     d_hlist.ensure(std::max<size_t>(nh, 1) * sizeof(uint32_t));
     HIPCHK(hipMemsetAsync(d_best.p, 0, (size_t)ns * sizeof(uint64_t), stream));
     if (nh) {
-      HIPCHK(hipMemcpyAsync(d_hlist.p, hslots.data(), nh * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+      h2d(d_hlist.p, hslots.data(), nh * sizeof(uint32_t));
       const uint32_t *dl = d_hlist.as<uint32_t>();
       if (nC <= 8) launch_split_t<8>(nh, dl, nT, nF, mode, all_gains);
       else if (nC <= 16) launch_split_t<16>(nh, dl, nT, nF, mode, all_gains);
@@ -749,9 +765,7 @@ struct Trainer final : TrainerBase {
                        d_res.as<uint32_t>());
     HIPCHK(hipGetLastError());
     res_out.resize((size_t)ns * (nC + 2));
-    HIPCHK(hipMemcpyAsync(res_out.data(), d_res.p, res_out.size() * sizeof(uint32_t), hipMemcpyDeviceToHost,
-                          stream));
-    HIPCHK(hipStreamSynchronize(stream));
+    d2h_sync(res_out.data(), d_res.p, res_out.size() * sizeof(uint32_t));
   }
 
   // network input of a node: class histogram n_c / n (16) and depth / 16
@@ -875,6 +889,7 @@ struct Trainer final : TrainerBase {
       split_slice(ns, hslots, nT, nFe, mode, res, gains);
       HIPCHK(hipEventRecord(e2.e, stream));
       HIPCHK(hipEventSynchronize(e2.e));
+      stage.reset();
       st.hist_ms += elapsed_ms(e0, e1);
       st.split_ms += elapsed_ms(e1, e2);
       if (emit) emit_records(sub, s0, ns, nFe, nT, bnd, tree, classTot, level, gains, res);
@@ -1107,20 +1122,16 @@ struct Trainer final : TrainerBase {
       }
       // upload this level's node range for routing
       const size_t lv0 = ((size_t)1 << (d - 1)) - 1, lvn = (size_t)1 << (d - 1);
-      HIPCHK(hipMemcpyAsync(d_tleft.as<int32_t>() + lv0, left + lv0, lvn * sizeof(int32_t), hipMemcpyHostToDevice,
-                            stream));
-      HIPCHK(hipMemcpyAsync(d_tfeat.as<T>() + lv0 * FD, feats + lv0 * FD, lvn * FD * sizeof(T),
-                            hipMemcpyHostToDevice, stream));
-      HIPCHK(hipMemcpyAsync(d_tthr.as<T>() + lv0, thrs + lv0, lvn * sizeof(T), hipMemcpyHostToDevice, stream));
+      h2d(d_tleft.as<int32_t>() + lv0, left + lv0, lvn * sizeof(int32_t));
+      h2d(d_tfeat.as<T>() + lv0 * FD, feats + lv0 * FD, lvn * FD * sizeof(T));
+      h2d(d_tthr.as<T>() + lv0, thrs + lv0, lvn * sizeof(T));
       if constexpr (uses_context<F>::value) {
         // the children (next level) as the hook will read them: leftChild -1
         // and their posteriors (the reference's device copy reads the
         // posteriors from the wrong offset, _learn_best_featthr.hpp:270-274)
         const size_t c0 = lv0 + lvn, cn = std::min(2 * lvn, N - c0);
-        HIPCHK(hipMemcpyAsync(d_tleft.as<int32_t>() + c0, left + c0, cn * sizeof(int32_t), hipMemcpyHostToDevice,
-                              stream));
-        HIPCHK(hipMemcpyAsync(d_tpost.as<float>() + c0 * nC, post + c0 * nC, cn * nC * sizeof(float),
-                              hipMemcpyHostToDevice, stream));
+        h2d(d_tleft.as<int32_t>() + c0, left + c0, cn * sizeof(int32_t));
+        h2d(d_tpost.as<float>() + c0 * nC, post + c0 * nC, cn * nC * sizeof(float));
         // every routed pixel one level down (context: the map before the step)
         hipLaunchKernelGGL((k_ac_route_map<F>), dim3(ac_blocks), dim3(256), 0, stream, d_pixels.as<I>(),
                            d_imgs.as<ImgDesc>(), n_images, d_acblk.as<uint32_t>(), d_tfeat.as<T>(), d_tthr.as<T>(),
@@ -1158,13 +1169,10 @@ struct Trainer final : TrainerBase {
       d_sbegin.ensure((nq + 1) * sizeof(uint32_t));
       d_flags.ensure((n_act + 1) * sizeof(unsigned long long));
       d_rank.ensure((n_act + 1) * sizeof(unsigned long long));
-      HIPCHK(hipMemcpyAsync(d_rslot_node.p, rnode.data(), nq * sizeof(int32_t), hipMemcpyHostToDevice, stream));
-      HIPCHK(hipMemcpyAsync(d_slot_next.p, snext.data(), 2 * (size_t)nq * sizeof(int32_t), hipMemcpyHostToDevice,
-                            stream));
-      HIPCHK(hipMemcpyAsync(d_sbegin.p, sbegin.data(), (nq + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
-      if (!nbegin.empty())
-        HIPCHK(hipMemcpyAsync(d_cursor.p, nbegin.data(), nbegin.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
-                              stream));
+      h2d(d_rslot_node.p, rnode.data(), nq * sizeof(int32_t));
+      h2d(d_slot_next.p, snext.data(), 2 * (size_t)nq * sizeof(int32_t));
+      h2d(d_sbegin.p, sbegin.data(), (nq + 1) * sizeof(uint32_t));
+      h2d(d_cursor.p, nbegin.data(), nbegin.size() * sizeof(uint32_t));
       size_t tmp_bytes = 0;
       HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, d_flags.as<unsigned long long>(),
                                               d_rank.as<unsigned long long>(), (int)(n_act + 1), stream));
@@ -1189,9 +1197,7 @@ struct Trainer final : TrainerBase {
       HIPCHK(hipGetLastError());
       HIPCHK(hipEventRecord(e4.e, stream));
       std::vector<uint32_t> ccount(2 * (size_t)nq);
-      HIPCHK(hipMemcpyAsync(ccount.data(), d_ccount.p, ccount.size() * sizeof(uint32_t), hipMemcpyDeviceToHost,
-                            stream));
-      HIPCHK(hipStreamSynchronize(stream));
+      d2h_sync(ccount.data(), d_ccount.p, ccount.size() * sizeof(uint32_t));
       st.route_ms = elapsed_ms(e3, e4);
       // the routed child populations must equal the split's class counts
       for (size_t q = 0; q < slots.size(); q++) {
@@ -1284,6 +1290,7 @@ struct Trainer final : TrainerBase {
     const size_t per_n = (size_t)nC * P.n_thresholds * P.n_features;
     HIPCHK(hipMemcpyAsync(H_out, d_H.p, per_n * n_nodes * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
+    stage.reset();
   }
 };
 

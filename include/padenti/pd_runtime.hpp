@@ -109,6 +109,56 @@ struct DevBuf {
   }
 };
 
+// Pinned staging for small per-step host<->device transfers.  An async copy
+// from pageable memory stages through the runtime's own buffers and blocks
+// the host; from pinned memory it is a plain DMA.  Space handed out since the
+// last reset() stays valid (and untouched) until the next reset(), which the
+// owner calls only after synchronizing the stream the copies ran on.
+struct PinnedStage {
+  char *base = nullptr;
+  size_t cap = 0, used = 0;
+  std::vector<char *> retired;   // outgrown blocks, freed at the next reset
+  PinnedStage() = default;
+  PinnedStage(const PinnedStage &) = delete;
+  PinnedStage &operator=(const PinnedStage &) = delete;
+  ~PinnedStage()
+  {
+    reset();
+    if (base) (void)hipHostFree(base);
+  }
+  void *alloc(size_t bytes)
+  {
+    const size_t need = (std::max<size_t>(bytes, 1) + 255) & ~(size_t)255;
+    if (used + need > cap) {
+      if (base) retired.push_back(base);
+      base = nullptr;
+      cap = std::max<size_t>(need, std::max<size_t>(2 * cap, (size_t)1 << 20));
+      used = 0;
+      if (hipHostMalloc((void **)&base, cap, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        base = nullptr;
+        cap = 0;
+        throw NoMem("hipHostMalloc of " + std::to_string(need) + " bytes failed");
+      }
+    }
+    void *r = base + used;
+    used += need;
+    return r;
+  }
+  void *put(const void *src, size_t bytes)
+  {
+    void *r = alloc(bytes);
+    if (bytes) std::copy((const char *)src, (const char *)src + bytes, (char *)r);
+    return r;
+  }
+  void reset()
+  {
+    used = 0;
+    for (char *b : retired) (void)hipHostFree(b);
+    retired.clear();
+  }
+};
+
 struct Event {
   hipEvent_t e = nullptr;
   Event() { HIPCHK(hipEventCreate(&e)); }
