@@ -1010,6 +1010,37 @@ __global__ __launch_bounds__(512) void k_lds_add(uint32_t iters, uint32_t *__res
   out[(size_t)blockIdx.x * blockDim.x + threadIdx.x] = bins[threadIdx.x];
 }
 
+// Texel-gather probe: every wave issues `iters` 64-lane uint16 loads whose
+// lanes fall into 64 / g distinct 128-byte lines (lanes grouped g by g),
+// the lines drawn at random from the first `lines` lines of `buf` (which
+// must hold lines * 128 bytes).  With the window inside the vector L1 it
+// times the L1-hit path, with a window of a few MiB the L1-miss (L2 fill)
+// path: the two ceilings of the histogram kernel's scattered gathers.
+// out: one word per thread (the sums, so no load is dead).
+__device__ __forceinline__ uint32_t probe_hash(uint32_t x)
+{
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+
+__global__ __launch_bounds__(256) void k_gather_probe(const uint16_t *__restrict__ buf, uint32_t lines, uint32_t g,
+                                                      uint32_t iters, uint32_t *__restrict__ out)
+{
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t grp = lane / g;
+  uint32_t acc = 0;
+  for (uint32_t it = 0; it < iters; it++) {
+    const uint32_t line = probe_hash(wave * 7919u + it * 104729u + grp * 31u) % lines;
+    acc += buf[line * 64u + ((lane * 5u + it) & 63u)];
+  }
+  out[blockIdx.x * blockDim.x + threadIdx.x] = acc;
+}
+
 __global__ void k_log2_det(const float *__restrict__ x, float *__restrict__ y, uint32_t n)
 {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;

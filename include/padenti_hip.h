@@ -233,6 +233,13 @@ PD_API int pd_forest_predict_frames_dev(pd_forest *fo, uint32_t n_frames, const 
 /* HBM probe: average time of a plain device-to-device copy kernel moving
  * `bytes` (read) + `bytes` (write), over `reps` launches. */
 PD_API int pd_copy_bench(int device, uint64_t bytes, int reps, float *ms_per_copy);
+/* Texel-gather probe: 32 waves per CU each issue `iters` 64-lane uint16
+ * loads touching `lines_per_load` distinct 128-byte lines, drawn at random
+ * from a window of `window_bytes` (inside the vector L1: the L1-hit rate;
+ * a few MiB: the L1-miss rate).  Reports ms per launch and CU cycles per
+ * wave load (at the device's reported clock). */
+PD_API int pd_gather_bench(int device, uint64_t window_bytes, uint32_t lines_per_load, uint32_t iters, int reps,
+                           float *ms_per_launch, double *cycles_per_load);
 
 /* LDS-atomic probe: 16 blocks of 512 threads per CU, each lane doing
  * 8 * iters conflict-free 32-bit LDS adds into its own table column (the

@@ -164,6 +164,7 @@ def lib():
         "pd_forest_predict_frames_dev": (I32, [VP, U32, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, I32,
                                                C.POINTER(C.c_float)]),
         "pd_copy_bench": (I32, [I32, U64, I32, C.POINTER(C.c_float)]),
+        "pd_gather_bench": (I32, [I32, U64, U32, U32, I32, C.POINTER(C.c_float), C.POINTER(C.c_double)]),
         "pd_lds_atomic_bench": (I32, [I32, U32, I32, C.POINTER(C.c_float), C.POINTER(C.c_double)]),
         "pd_mlp_create": (I32, [C.POINTER(VP), I32, U64]),
         "pd_mlp_destroy": (I32, [VP]),

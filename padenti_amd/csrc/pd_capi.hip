@@ -383,6 +383,44 @@ int pd_lds_atomic_bench(int device, uint32_t iters, int reps, float *ms_per_laun
   });
 }
 
+int pd_gather_bench(int device, uint64_t window_bytes, uint32_t lines_per_load, uint32_t iters, int reps,
+                    float *ms_per_launch, double *cycles_per_load)
+{
+  return guarded([&] {
+    require(window_bytes >= 128 && window_bytes <= (1ull << 30) && window_bytes % 128 == 0,
+            "window must be a multiple of 128 bytes in [128 B, 1 GiB]");
+    require(lines_per_load >= 1 && lines_per_load <= 64 && 64 % lines_per_load == 0,
+            "lines_per_load must divide 64");
+    require(iters >= 1 && iters <= (1u << 16) && reps >= 1 && ms_per_launch && cycles_per_load,
+            "bad gather-bench arguments");
+    set_device(device);
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    const uint32_t cus = (uint32_t)prop.multiProcessorCount;
+    const uint32_t blocks = cus * 8;   // 32 waves per CU
+    const uint32_t lines = (uint32_t)(window_bytes / 128);
+    DevBuf buf, out;
+    buf.ensure(window_bytes);
+    HIPCHK(hipMemset(buf.p, 1, window_bytes));
+    out.ensure((size_t)blocks * 256 * sizeof(uint32_t));
+    const uint32_t g = 64u / lines_per_load;
+    hipLaunchKernelGGL(k_gather_probe, dim3(blocks), dim3(256), 0, nullptr, buf.as<uint16_t>(), lines, g, 64u,
+                       out.as<uint32_t>());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+    Event e0, e1;
+    HIPCHK(hipEventRecord(e0.e, nullptr));
+    for (int r = 0; r < reps; r++)
+      hipLaunchKernelGGL(k_gather_probe, dim3(blocks), dim3(256), 0, nullptr, buf.as<uint16_t>(), lines, g, iters,
+                         out.as<uint32_t>());
+    HIPCHK(hipEventRecord(e1.e, nullptr));
+    HIPCHK(hipEventSynchronize(e1.e));
+    *ms_per_launch = (float)(elapsed_ms(e0, e1) / reps);
+    const double loads_per_cu = (double)blocks * 4.0 * (double)iters / (double)cus;
+    *cycles_per_load = (double)*ms_per_launch * 1e-3 * (double)prop.clockRate * 1e3 / loads_per_cu;
+  });
+}
+
 int pd_dev_alloc(int device, uint64_t bytes, void **out)
 {
   return guarded([&] {
