@@ -388,7 +388,7 @@ struct Trainer final : TrainerBase {
   DevBuf d_pixels, d_imgs, d_recs;
   // level state
   DevBuf d_actA, d_actB, d_slotA, d_slotB;
-  DevBuf d_H, d_items, d_slot_node, d_slot_mode, d_slot_hidx, d_zero, d_tot, d_best, d_res;
+  DevBuf d_H, d_items, d_slot_node, d_slot_mode, d_slot_hidx, d_zero, d_tot, d_slot_h, d_best, d_res;
   DevBuf d_cand_key, d_cand_left, d_hlist;
   DevBuf d_tleft, d_tfeat, d_tthr, d_rslot_node, d_slot_next, d_cursor, d_ccount;
   DevBuf d_flags, d_rank, d_sbegin, d_scan_tmp;
@@ -583,7 +583,8 @@ struct Trainer final : TrainerBase {
                                  (int)lds));
     hipLaunchKernelGGL((k_hist<F, NTP, NW>), dim3(n_items), dim3(NW * 64), lds, stream, d_pixels.as<I>(),
                        d_imgs.as<ImgDesc>(), act, d_items.as<HistItem>(), item_base, d_slot_node.as<int32_t>(),
-                       d_slot_mode.as<uint8_t>(), d_slot_hidx.as<int32_t>(), d_tot.as<uint32_t>(), bnd, tree, nF,
+                       d_slot_mode.as<uint8_t>(), d_slot_hidx.as<int32_t>(), d_tot.as<uint32_t>(),
+                       d_slot_h.as<float>(), bnd, tree, nF,
                        nT, nC, d_H.as<uint32_t>(), d_cand_key.as<unsigned long long>(), d_cand_left.as<uint32_t>(),
                        d_map_cur, d_perm_cur, ac);
     HIPCHK(hipGetLastError());
@@ -677,6 +678,10 @@ struct Trainer final : TrainerBase {
     h2d(d_slot_mode.p, smode.data(), ns);
     h2d(d_slot_hidx.p, shidx.data(), ns * sizeof(int32_t));
     h2d(d_tot.p, tot.data(), (size_t)ns * nC * sizeof(uint32_t));
+    std::vector<float> sh(ns);
+    for (uint32_t q = 0; q < ns; q++) sh[q] = slot_entropy(&tot[(size_t)q * nC], nC);
+    d_slot_h.ensure(ns * sizeof(float));
+    h2d(d_slot_h.p, sh.data(), ns * sizeof(float));
     if (!zero.empty()) {
       d_zero.ensure(zero.size() * sizeof(uint32_t));
       h2d(d_zero.p, zero.data(), zero.size() * sizeof(uint32_t));

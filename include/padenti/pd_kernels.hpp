@@ -255,7 +255,8 @@ __global__ __launch_bounds__(NWAVES * 64) void k_hist(
     const typename F::ImgType *__restrict__ pixels, const ImgDesc *__restrict__ imgs,
     const uint4 *__restrict__ recs, const HistItem *__restrict__ items, uint32_t item_base,
     const int32_t *__restrict__ slot_node, const uint8_t *__restrict__ slot_mode,
-    const int32_t *__restrict__ slot_hidx, const uint32_t *__restrict__ tot, Bounds<F> bnd, uint32_t tree,
+    const int32_t *__restrict__ slot_hidx, const uint32_t *__restrict__ tot, const float *__restrict__ slot_h,
+    Bounds<F> bnd, uint32_t tree,
     uint32_t nF, uint32_t nT, uint32_t nC, uint32_t *__restrict__ H, unsigned long long *__restrict__ cand_key,
     uint32_t *__restrict__ cand_left, const uint32_t *__restrict__ cand_map, const uint16_t *__restrict__ perm,
     AcArgs ac)
@@ -394,8 +395,9 @@ __global__ __launch_bounds__(NWAVES * 64) void k_hist(
   int N = 0;
   for (uint32_t c = 0; c < nC; c++) N += (int)tt[c];
   const float fN = (float)N;
-  float h = 0.0f;
-  for (uint32_t c = 0; c < nC; c++) h -= plogp((float)tt[c] / fN);
+  // the node entropy -sum plogp(tot_c / N), computed once per slot by the
+  // host with the same FP32 / log2 operations (slot_entropy)
+  const float h = slot_h[it.slot];
   unsigned long long best = 0ull;
   if (fv) {
     for (uint32_t t = wave; t < nT; t += NW) {

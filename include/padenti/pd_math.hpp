@@ -214,6 +214,18 @@ PD_HD float log2_det(float x)
 // literal 1.e-6, so the float is compared after promotion).
 PD_HD float plogp(float tmp) { return ((double)tmp <= 1.e-6) ? 0.0f : tmp * log2_det(tmp); }
 
+// Node entropy h = -sum_c plogp(tot_c / N), in the order learnBestFeature
+// accumulates it (learn_best_feature.cl:63-76).
+PD_HD float slot_entropy(const uint32_t *tot, uint32_t nC)
+{
+  uint32_t N = 0;
+  for (uint32_t c = 0; c < nC; c++) N += tot[c];
+  const float fN = (float)(int)N;
+  float h = 0.0f;
+  for (uint32_t c = 0; c < nC; c++) h -= plogp((float)tot[c] / fN);
+  return h;
+}
+
 // ---------------------------------------------------------------------------
 // FP32 information gain of one split, op for op as learnBestFeature
 // (learn_best_feature.cl:42-93): left[c] = histogram count, tot[c] = node's
