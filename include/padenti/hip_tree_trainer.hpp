@@ -159,6 +159,12 @@ public:
   {
     pd_cxx::check(pd_trainer_set_proposal(m_tr, net.handle(), keep, screen, minSamples, judge ? 1 : 0));
   }
+  // Refinement of the proposal: the screened candidates ranked by their exact
+  // gain on every stride-th sample (at least minSamples); stride 0 turns it off.
+  void setProposalRefine(uint32_t stride, uint32_t minSamples)
+  {
+    pd_cxx::check(pd_trainer_set_proposal_refine(m_tr, stride, minSamples));
+  }
   ~HIPTreeTrainer()
   {
     if (m_tr) pd_trainer_destroy(m_tr);

@@ -352,6 +352,7 @@ struct TrainerBase {
   // split proposal (PD_MODE_PROPOSAL)
   MlpNet *prop_mlp = nullptr;
   uint32_t prop_keep = 0, prop_screen = 0, prop_min = 0;
+  uint32_t prop_refine = 0, prop_refine_min = 0;   // subsample stride (0: off), smallest subsample
   int prop_judge = 0;
   virtual ~TrainerBase()
   {
@@ -421,6 +422,7 @@ struct Trainer final : TrainerBase {
     if (bytes) std::copy((const char *)pin, (const char *)pin + bytes, (char *)dst);
     stage.reset();
   }
+  DevBuf d_sub_act, d_sub_idx, d_sub_cls, d_sub_sel, d_map2;
   DevBuf d_gains, d_emit_slots, d_emit_base, d_emit_best, d_emit_in, d_emit_node, d_pslot_in, d_pslot_node, d_scores,
       d_map;
 
@@ -910,6 +912,102 @@ struct Trainer final : TrainerBase {
     }
   }
 
+  // Refinement stage of the split proposal for the slots gidx: each slot's
+  // `screen` candidates (d_map1 [G][screen] / h_map1, sampler indices) are
+  // evaluated exactly on an evenly spaced subsample of its samples (every
+  // prop_refine-th, at least prop_refine_min, all samples of a smaller
+  // node), and the `keep` with the largest subsample gains (ties to the
+  // lower position, written in position order) form h_map2 / d_map2.
+  void refine(const std::vector<SlotInfo> &slots, const std::vector<uint32_t> &gidx, const uint4 *act,
+              const pd_train_params &P, uint32_t screen, uint32_t keep, const uint32_t *d_map1,
+              const std::vector<uint32_t> &h_map1, const Bounds<F> &bnd, uint32_t tree,
+              std::vector<uint32_t> &h_map2, pd_level_stats &st)
+  {
+    const uint32_t G = (uint32_t)gidx.size();
+    std::vector<uint32_t> sb(G), sc(G), db(G + 1), dc(G);
+    uint64_t total = 0;
+    for (uint32_t g = 0; g < G; g++) {
+      const SlotInfo &si = slots[gidx[g]];
+      sb[g] = si.begin;
+      sc[g] = si.count;
+      const uint32_t stride = std::max<uint32_t>(prop_refine, 1u);
+      const uint32_t want = std::max<uint32_t>(prop_refine_min, (si.count + stride - 1) / stride);
+      dc[g] = std::min<uint32_t>(si.count, want);
+      db[g] = (uint32_t)total;
+      total += dc[g];
+    }
+    db[G] = (uint32_t)total;
+    require(total < (1ull << 32), "refinement subsample too large");
+    Event ea, eb;
+    HIPCHK(hipEventRecord(ea.e, stream));
+    d_sub_idx.ensure((size_t)(4 * G + 1) * sizeof(uint32_t));
+    uint32_t *di = d_sub_idx.as<uint32_t>();
+    h2d(di, sb.data(), G * sizeof(uint32_t));
+    h2d(di + G, sc.data(), G * sizeof(uint32_t));
+    h2d(di + 2 * G, db.data(), (G + 1) * sizeof(uint32_t));
+    h2d(di + 3 * G + 1, dc.data(), G * sizeof(uint32_t));
+    d_sub_act.ensure(std::max<uint64_t>(total, 1) * sizeof(uint4));
+    d_sub_cls.ensure((size_t)G * nC * sizeof(uint32_t));
+    HIPCHK(hipMemsetAsync(d_sub_cls.p, 0, (size_t)G * nC * sizeof(uint32_t), stream));
+    if (total) {
+      hipLaunchKernelGGL(k_subsample, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, stream, act, di, di + G,
+                         di + 2 * G, di + 3 * G + 1, G, (uint32_t)total, nC, d_sub_act.as<uint4>(),
+                         d_sub_cls.as<uint32_t>());
+      HIPCHK(hipGetLastError());
+    }
+    std::vector<uint32_t> cls((size_t)G * nC);
+    d2h_sync(cls.data(), d_sub_cls.p, cls.size() * sizeof(uint32_t));
+    // the subsample as frontier slots with their own class totals, searched
+    // with every slot's H block and the gains of all its pairs
+    std::vector<SlotInfo> sub(G);
+    for (uint32_t g = 0; g < G; g++) sub[g] = SlotInfo{slots[gidx[g]].node, db[g], dc[g], true};
+    pd_train_params Pe = P;
+    Pe.n_features = screen;
+    Pe.n_thresholds = 1;
+    const uint32_t ch = pick_chunk(total, (screen + 63) / 64);
+    const size_t per_bytes = (size_t)nC * screen * sizeof(uint32_t);
+    const size_t h_per_slice = std::max<size_t>(
+        1, std::min<size_t>(hist_budget / per_bytes, (1ull << 30) / ((size_t)screen * 4)));
+    h_map2.assign((size_t)G * keep, 0u);
+    for (size_t s0 = 0; s0 < G; s0 += h_per_slice) {
+      const size_t s1 = std::min<size_t>(G, s0 + h_per_slice);
+      const uint32_t ns = (uint32_t)(s1 - s0);
+      std::vector<uint32_t> tot(cls.begin() + s0 * nC, cls.begin() + s1 * nC), hslots, res;
+      d_map_cur = d_map1 + s0 * screen;
+      hist_slice(sub, s0, s1, d_sub_act.as<uint4>(), Pe, bnd, tree, PD_MODE_LEVEL_BATCHED, ch, tot, true, hslots);
+      d_map_cur = nullptr;
+      d_gains.ensure((size_t)ns * screen * sizeof(float));
+      // the gains of every (slot, candidate) pair of the slice
+      {
+        const uint32_t nh = (uint32_t)hslots.size();
+        d_best.ensure((size_t)ns * sizeof(uint64_t));
+        d_hlist.ensure(std::max<size_t>(nh, 1) * sizeof(uint32_t));
+        HIPCHK(hipMemsetAsync(d_best.p, 0, (size_t)ns * sizeof(uint64_t), stream));
+        h2d(d_hlist.p, hslots.data(), nh * sizeof(uint32_t));
+        const uint32_t *dl = d_hlist.as<uint32_t>();
+        if (nC <= 8) launch_split_t<8>(nh, dl, 1, screen, PD_MODE_LEVEL_BATCHED, d_gains.as<float>());
+        else if (nC <= 16) launch_split_t<16>(nh, dl, 1, screen, PD_MODE_LEVEL_BATCHED, d_gains.as<float>());
+        else if (nC <= 32) launch_split_t<32>(nh, dl, 1, screen, PD_MODE_LEVEL_BATCHED, d_gains.as<float>());
+        else launch_split_t<64>(nh, dl, 1, screen, PD_MODE_LEVEL_BATCHED, d_gains.as<float>());
+      }
+      d_sub_sel.ensure((size_t)ns * keep * sizeof(uint32_t));
+      hipLaunchKernelGGL(mlp::k_topk, dim3(ns), dim3(256), 0, stream, d_gains.as<float>(), screen, keep, keep,
+                         d_sub_sel.as<uint32_t>());
+      HIPCHK(hipGetLastError());
+      std::vector<uint32_t> sel((size_t)ns * keep);
+      d2h_sync(sel.data(), d_sub_sel.p, sel.size() * sizeof(uint32_t));
+      for (uint32_t q = 0; q < ns; q++)
+        for (uint32_t j = 0; j < keep; j++)
+          h_map2[(s0 + q) * keep + j] = h_map1[(s0 + q) * screen + sel[(size_t)q * keep + j]];
+    }
+    d_map2.ensure(h_map2.size() * sizeof(uint32_t));
+    h2d(d_map2.p, h_map2.data(), h_map2.size() * sizeof(uint32_t));
+    HIPCHK(hipEventRecord(eb.e, stream));
+    HIPCHK(hipEventSynchronize(eb.e));
+    stage.reset();
+    st.refine_ms += elapsed_ms(ea, eb);
+  }
+
   // Split proposal of the slots idx: the network scores the pool of
   // K = P.n_features candidates of every node (one threshold each), the
   // best `prop_keep` are histogrammed exactly; optionally the exhaustive
@@ -939,7 +1037,6 @@ struct Trainer final : TrainerBase {
       d_pslot_in.ensure(sin.size() * sizeof(float));
       d_pslot_node.ensure(G * sizeof(int32_t));
       d_scores.ensure(rows * sizeof(float));
-      d_map.ensure(G * keep * sizeof(uint32_t));
       HIPCHK(hipMemcpyAsync(d_pslot_in.p, sin.data(), sin.size() * sizeof(float), hipMemcpyHostToDevice, stream));
       HIPCHK(hipMemcpyAsync(d_pslot_node.p, snode.data(), G * sizeof(int32_t), hipMemcpyHostToDevice, stream));
       const PoolRows<F> src{d_pslot_in.as<float>(), d_pslot_node.as<int32_t>(), K, tree, (uint32_t)rows, bnd, nm};
@@ -951,16 +1048,26 @@ struct Trainer final : TrainerBase {
       HIPCHK(hipGetLastError());
       HIPCHK(hipEventRecord(eb.e, stream));
       const uint32_t screen = std::min<uint32_t>(K, std::max<uint32_t>(keep, prop_screen));
-      hipLaunchKernelGGL(mlp::k_topk, dim3((uint32_t)G), dim3(256), 0, stream, d_scores.as<float>(), K, screen, keep,
+      // with refinement the whole screened set is kept for the subsample stage
+      const uint32_t first = prop_refine ? screen : keep;
+      d_map.ensure(G * first * sizeof(uint32_t));
+      hipLaunchKernelGGL(mlp::k_topk, dim3((uint32_t)G), dim3(256), 0, stream, d_scores.as<float>(), K, screen, first,
                          d_map.as<uint32_t>());
       HIPCHK(hipGetLastError());
       HIPCHK(hipEventRecord(ec.e, stream));
-      std::vector<uint32_t> hmap(G * keep);
+      std::vector<uint32_t> hmap(G * first);
       HIPCHK(hipMemcpyAsync(hmap.data(), d_map.p, hmap.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
       HIPCHK(hipStreamSynchronize(stream));
       st.mlp_ms += elapsed_ms(ea, eb);
       st.select_ms += elapsed_ms(eb, ec);
-      evaluate(slots, gidx, act, P, keep, d_map.as<uint32_t>(), &hmap, bnd, tree, PD_MODE_LEVEL_BATCHED,
+      const uint32_t *dmap = d_map.as<uint32_t>();
+      if (prop_refine && keep < screen) {
+        std::vector<uint32_t> hmap2;
+        refine(slots, gidx, act, P, screen, keep, d_map.as<uint32_t>(), hmap, bnd, tree, hmap2, st);
+        hmap.swap(hmap2);
+        dmap = d_map2.as<uint32_t>();
+      }
+      evaluate(slots, gidx, act, P, keep, dmap, &hmap, bnd, tree, PD_MODE_LEVEL_BATCHED,
                pick_chunk(gact, (keep + 63) / 64), classTot, level, false, res_all, st);
       if (prop_judge) {
         std::vector<uint32_t> res_j(slots.size() * (nC + 2), 0u);

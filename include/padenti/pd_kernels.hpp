@@ -935,6 +935,33 @@ __global__ __launch_bounds__(256) void k_fill_i32(int32_t *__restrict__ p, uint6
   for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256u) p[i] = v;
 }
 
+// Evenly spaced subsample of every slot's records (the refinement stage of
+// the split proposal): slot q's dst_count[q] records at dst_begin[q] are
+// src[src_begin[q] + floor(j * src_count[q] / dst_count[q])], j < dst_count[q]
+// (dst_begin has n_slots + 1 entries, the last = total); their classes are
+// counted into cls[q][nC] (zeroed by the caller).
+__global__ __launch_bounds__(256) void k_subsample(const uint4 *__restrict__ src, const uint32_t *__restrict__ src_begin,
+                                                   const uint32_t *__restrict__ src_count,
+                                                   const uint32_t *__restrict__ dst_begin,
+                                                   const uint32_t *__restrict__ dst_count, uint32_t n_slots,
+                                                   uint32_t total, uint32_t nC, uint4 *__restrict__ dst,
+                                                   uint32_t *__restrict__ cls)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  // the slot: dst_begin[lo] <= i < dst_begin[lo + 1]
+  uint32_t lo = 0, hi = n_slots;
+  while (hi - lo > 1u) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (dst_begin[mid] <= i) lo = mid;
+    else hi = mid;
+  }
+  const uint32_t j = i - dst_begin[lo];
+  const uint4 r = src[src_begin[lo] + (uint32_t)(((uint64_t)j * src_count[lo]) / dst_count[lo])];
+  dst[i] = r;
+  atomicAdd(&cls[(size_t)lo * nC + rec_cls(r)], 1u);
+}
+
 // ---------------------------------------------------------------------------
 // Small unit kernels used by the parity tests of the individual steps.
 // ---------------------------------------------------------------------------

@@ -93,6 +93,7 @@ typedef struct {
   double gain_proposal;     /* sum over judged nodes of the chosen split's FP32 gain */
   double gain_loss_rel;     /* sum over judged nodes of (g_exh - g_prop) / g_exh, g_exh > 0 */
   uint32_t reserved;
+  double refine_ms;         /* PD_MODE_PROPOSAL with refinement: device time of the subsample ranking */
 } pd_level_stats;
 
 /* Launch tuning of the histogram kernel (chosen by the autotuner). */
@@ -301,6 +302,14 @@ PD_API int pd_trainer_records_written(pd_trainer *tr, uint64_t *n);
  * (pd_level_stats). */
 PD_API int pd_trainer_set_proposal(pd_trainer *tr, pd_mlp *mlp, uint32_t keep, uint32_t screen, uint32_t min_samples,
                                    int judge);
+/* Refinement of the proposal (no counterpart in the reference): the
+ * `screen` best-scored candidates of a node (screen from
+ * pd_trainer_set_proposal, or all of them) are evaluated exactly on an evenly
+ * spaced subsample of the node's samples (every `stride`-th, at least
+ * `min_samples`, all samples of a smaller node), and the `keep` with the
+ * largest subsample gains (ties to the lower sampler index) are histogrammed
+ * on all samples.  stride 0 turns it off. */
+PD_API int pd_trainer_set_proposal_refine(pd_trainer *tr, uint32_t stride, uint32_t min_samples);
 
 /* ---- dataset ingestion ---------------------------------------------------
  * Replaces (paths relative to the reference root):

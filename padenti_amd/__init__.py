@@ -240,6 +240,13 @@ class HIPTreeTrainer:
         check(lib().pd_trainer_set_proposal(self._h, net._h if net is not None else None, int(keep),
                                             int(screen or keep), int(min_samples), 1 if judge else 0))
 
+    def set_proposal_refine(self, stride, min_samples=0):
+        """Refinement of PD_MODE_PROPOSAL: the `screen` best-scored candidates
+        of a node are ranked by their exact gain on every stride-th sample (at
+        least min_samples) and the best `keep` are histogrammed; stride 0
+        turns it off."""
+        check(lib().pd_trainer_set_proposal_refine(self._h, int(stride), int(min_samples)))
+
     def train(self, tree, ts, params, start_depth, end_depth, mode=PD_MODE_LEVEL_BATCHED, diagnostics=False):
         """TreeTrainer::train(tree, ts, params, startDepth=1, endDepth)."""
         if start_depth != 1:

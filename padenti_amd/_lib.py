@@ -81,6 +81,7 @@ class LevelStats(C.Structure):
         ("gain_proposal", C.c_double),
         ("gain_loss_rel", C.c_double),
         ("reserved", C.c_uint32),
+        ("refine_ms", C.c_double),
     ]
 
     def as_dict(self):
@@ -182,6 +183,7 @@ def lib():
         "pd_trainer_set_record_sink": (I32, [VP, VP, VP, VP, U64, U32]),
         "pd_trainer_records_written": (I32, [VP, C.POINTER(U64)]),
         "pd_trainer_set_proposal": (I32, [VP, VP, U32, U32, U32, I32]),
+        "pd_trainer_set_proposal_refine": (I32, [VP, U32, U32]),
         "pd_dev_alloc": (I32, [I32, U64, C.POINTER(VP)]),
         "pd_dev_free": (I32, [VP]),
         "pd_memcpy_h2d": (I32, [VP, VP, U64]),

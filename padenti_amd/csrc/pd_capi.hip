@@ -648,6 +648,15 @@ int pd_trainer_records_written(pd_trainer *tr, uint64_t *n)
   });
 }
 
+int pd_trainer_set_proposal_refine(pd_trainer *tr, uint32_t stride, uint32_t min_samples)
+{
+  return guarded([&] {
+    require(tr != nullptr, "null trainer");
+    tr->impl->prop_refine = stride;
+    tr->impl->prop_refine_min = min_samples;
+  });
+}
+
 int pd_trainer_set_proposal(pd_trainer *tr, pd_mlp *m, uint32_t keep, uint32_t screen, uint32_t min_samples,
                             int judge)
 {

@@ -19,6 +19,11 @@
    level against the exhaustive pool; the judge also runs for the screened
    selection (keep taken in sampler order from the best --screen scores) and
    for constant scores (the first 2000 candidates: no network).
+   refined:    the network's best --refine-screen candidates of a node
+               (default: the whole pool) are ranked by their exact gain on
+               every --refine-stride-th sample (at least --refine-min) and the
+               best 2000 histogrammed; with a screen below the pool the same
+               refinement of the whole pool runs beside it.
 4. Held-out pixel accuracy of each tree on 20 held-out images.
 Prints per-level lines on stderr and one JSON line.
 """
@@ -81,6 +86,13 @@ def parser():
     ap.add_argument("--screen", type=int, default=10000,
                     help="also judge the screened selection: keep taken in sampler order from the best `screen`")
     ap.add_argument("--depth", type=int, default=16)
+    ap.add_argument("--refine-screen", type=int, default=0,
+                    help="refinement: the network's best `refine-screen` candidates of a node (0: the whole "
+                         "pool) ...")
+    ap.add_argument("--refine-stride", type=int, default=16,
+                    help="... are ranked by their exact gain on every refine-stride-th sample ...")
+    ap.add_argument("--refine-min", type=int, default=1024,
+                    help="... (at least refine-min samples) and the best `keep` histogrammed")
     return ap
 
 
@@ -128,6 +140,22 @@ def run(a):
     assert np.array_equal(judged_tree.left_children, t_prop_tree.left_children)
     tr.set_proposal(net, keep=a.keep, min_samples=a.min_samples, judge=True, screen=a.screen)
     screened_tree, t_screen, st_screen = timed_train(tr, ts, pool, a.depth, pa.PD_MODE_PROPOSAL)
+    # refinement: the network's best candidates ranked exactly on a subsample;
+    # beside it the same refinement of the whole pool (no use of the network)
+    rscreen = a.refine_screen or a.pool
+    tr.set_proposal_refine(a.refine_stride, a.refine_min)
+    tr.set_proposal(net, keep=a.keep, min_samples=a.min_samples, judge=False, screen=rscreen)
+    ref_tree, t_ref, st_ref = timed_train(tr, ts, pool, a.depth, pa.PD_MODE_PROPOSAL)
+    tr.set_proposal(net, keep=a.keep, min_samples=a.min_samples, judge=True, screen=rscreen)
+    _, _, st_ref_j = timed_train(tr, ts, pool, a.depth, pa.PD_MODE_PROPOSAL)
+    if rscreen < a.pool:
+        tr.set_proposal(net, keep=a.keep, min_samples=a.min_samples, judge=False, screen=a.pool)
+        refall_tree, t_refall, _ = timed_train(tr, ts, pool, a.depth, pa.PD_MODE_PROPOSAL)
+        tr.set_proposal(net, keep=a.keep, min_samples=a.min_samples, judge=True, screen=a.pool)
+        _, _, st_refall_j = timed_train(tr, ts, pool, a.depth, pa.PD_MODE_PROPOSAL)
+    else:
+        refall_tree, t_refall, st_refall_j = ref_tree, t_ref, st_ref_j
+    tr.set_proposal_refine(0, 0)
     t_small_tree, t_small, _ = timed_train(tr, ts, small, a.depth, pa.PD_MODE_LEVEL_BATCHED)
     net.params = np.zeros_like(trained)   # constant scores: the first `keep` candidates
     tr.set_proposal(net, keep=a.keep, min_samples=a.min_samples, judge=True)
@@ -138,11 +166,14 @@ def run(a):
         return sum(s["gain_loss_rel"] for s in st) / max(n, 1)
 
     levels = []
-    for sp, sj, sc, se in zip(st_prop, st_judge, st_const, st_exh):
+    for sp, sj, sc, se, sr, srj in zip(st_prop, st_judge, st_const, st_exh, st_ref, st_ref_j):
         n = sj["judged_nodes"]
         row = {"level": sp["level"], "proposal_nodes": sp["proposal_nodes"], "fallback_nodes": sp["fallback_nodes"],
                "gain_loss": sj["gain_loss_rel"] / n if n else None,
                "gain_loss_no_network": sc["gain_loss_rel"] / sc["judged_nodes"] if sc["judged_nodes"] else None,
+               "gain_loss_refined": srj["gain_loss_rel"] / srj["judged_nodes"] if srj["judged_nodes"] else None,
+               "refined_ms": sr["hist_ms"] + sr["split_ms"] + sr["mlp_ms"] + sr["select_ms"] + sr["refine_ms"],
+               "refine_ms": sr["refine_ms"],
                "exhaustive_ms": se["hist_ms"] + se["split_ms"],
                "proposal_ms": sp["hist_ms"] + sp["split_ms"] + sp["mlp_ms"] + sp["select_ms"],
                "mlp_ms": sp["mlp_ms"], "select_ms": sp["select_ms"]}
@@ -151,9 +182,12 @@ def run(a):
             f"gain loss {row['gain_loss'] if row['gain_loss'] is not None else float('nan'):.4f} "
             f"(no network {row['gain_loss_no_network'] if row['gain_loss_no_network'] is not None else float('nan'):.4f}) "
             f"exhaustive {row['exhaustive_ms']:8.2f} ms proposal {row['proposal_ms']:8.2f} ms "
-            f"(mlp {row['mlp_ms']:.2f}, select {row['select_ms']:.2f})")
+            f"(mlp {row['mlp_ms']:.2f}, select {row['select_ms']:.2f}); refined: gain loss "
+            f"{row['gain_loss_refined'] if row['gain_loss_refined'] is not None else float('nan'):.4f} "
+            f"{row['refined_ms']:8.2f} ms (subsample ranking {row['refine_ms']:.2f})")
     acc = {"exhaustive": accuracy(t_exh_tree, kind, nC, held), "proposal": accuracy(t_prop_tree, kind, nC, held),
            "screened": accuracy(screened_tree, kind, nC, held),
+           "refined": accuracy(ref_tree, kind, nC, held), "refined_whole_pool": accuracy(refall_tree, kind, nC, held),
            "first_keep_no_network": accuracy(t_small_tree, kind, nC, held)}
     rows_scored = sum(s["proposal_nodes"] for s in st_prop) * a.pool
     mlp_s = sum(s["mlp_ms"] for s in st_prop) / 1e3
@@ -169,6 +203,12 @@ def run(a):
         "mean_gain_loss": gl(st_judge),
         "mean_gain_loss_screened": gl(st_screen), "screen": a.screen,
         "mean_gain_loss_no_network": gl(st_const),
+        "refined": {"screen": rscreen, "stride": a.refine_stride, "min_samples": a.refine_min,
+                    "tree_s": t_ref, "speedup": t_exh / t_ref, "mean_gain_loss": gl(st_ref_j),
+                    "accuracy_loss_pt": 100 * (acc["exhaustive"] - acc["refined"]),
+                    "whole_pool": {"tree_s": t_refall, "speedup": t_exh / t_refall,
+                                   "mean_gain_loss": gl(st_refall_j),
+                                   "accuracy_loss_pt": 100 * (acc["exhaustive"] - acc["refined_whole_pool"])}},
         "heldout_pixel_accuracy": acc,
         "accuracy_loss_pt": 100 * (acc["exhaustive"] - acc["proposal"]),
         "records": n_rec, "record_s": rec_s,

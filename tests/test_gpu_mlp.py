@@ -204,6 +204,40 @@ def test_proposal_keep_all_equals_plain(gpu):
     net.close()
 
 
+def test_proposal_refine_full_sample_equals_plain(gpu):
+    """Refinement over every sample (stride 1) of the whole pool ranks the
+    candidates by their exact gains, so the kept quarter holds each node's
+    best split and the tree is the plain tree over the whole pool (also
+    with the minimum subsample covering small nodes, stride 4)."""
+    ts = synth.training_set("c2", n_images=10, samples=400)
+    K = 256
+    p = _pool_params(K)
+    tr = pa.HIPTreeTrainer(pa.PD_FEAT_DEPTH_U16, 8)
+    plain = pa.Tree(0, 9, pa.PD_FEAT_DEPTH_U16, 8)
+    tr.train(plain, ts, p, 1, 9)
+    net = pp.SplitProposalNet(seed=21)
+    for stride, min_sub in ((1, 0), (4, 1 << 20)):
+        tr.set_proposal(net, keep=K // 4, min_samples=0, judge=True, screen=K)
+        tr.set_proposal_refine(stride, min_sub)
+        prop = pa.Tree(0, 9, pa.PD_FEAT_DEPTH_U16, 8)
+        tr.train(prop, ts, p, 1, 9, mode=pa.PD_MODE_PROPOSAL)
+        assert np.array_equal(prop.left_children, plain.left_children), stride
+        assert prop.features.tobytes() == plain.features.tobytes()
+        assert prop.thresholds.tobytes() == plain.thresholds.tobytes()
+        st = tr.last_level_stats
+        assert sum(s["proposal_nodes"] for s in st) > 10
+        assert all(s["gain_loss_rel"] == 0.0 for s in st)
+        assert sum(s["refine_ms"] for s in st) > 0.0
+    # a real subsample (every 4th sample of nodes above 64) keeps most of the gain
+    tr.set_proposal_refine(4, 64)
+    prop = pa.Tree(0, 9, pa.PD_FEAT_DEPTH_U16, 8)
+    tr.train(prop, ts, p, 1, 9, mode=pa.PD_MODE_PROPOSAL)
+    assert _mean_gain_loss(tr.last_level_stats) < 0.05
+    tr.set_proposal_refine(0, 0)
+    tr.close()
+    net.close()
+
+
 def _mean_gain_loss(st):
     n = sum(s["judged_nodes"] for s in st)
     return sum(s["gain_loss_rel"] for s in st) / max(n, 1)
