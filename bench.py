@@ -110,8 +110,9 @@ def extras():
       predict:  256 mixed RGB-D frames (76.2 Mpx) through 5 depth-20 trees in
                 one hipGraph launch (padenti_amd/predict_bench.py);
       proposal: learned split proposal vs exhaustive evaluation of a
-                20000-candidate pool on the same data
-                (padenti_amd/proposal_bench.py);
+                20000-candidate pool on the same data, plain (network top
+                2000) and refined (candidates ranked on a subsample of each
+                node) (padenti_amd/proposal_bench.py);
       ingest:   uniform and class-weighted sampling of 5000 640x480 uint16
                 images resident in HBM (padenti_amd/ingest_bench.py), with
                 the CPU oracle (oracle/ingest.c, images in parallel over the
@@ -131,8 +132,9 @@ def extras():
         res["proposal"] = {k: r[k] for k in ("value", "unit", "exhaustive_tree_s", "proposal_tree_s", "pool", "keep",
                                              "mean_gain_loss", "mean_gain_loss_screened",
                                              "mean_gain_loss_no_network", "heldout_pixel_accuracy",
-                                             "accuracy_loss_pt", "records", "network_train_s")}
-        print(f"[bench] proposal {r['value']:.2f}x over exhaustive, gain loss {r['mean_gain_loss']:.3f}",
+                                             "accuracy_loss_pt", "records", "network_train_s", "refined")}
+        print(f"[bench] proposal {r['value']:.2f}x over exhaustive, gain loss {r['mean_gain_loss']:.3f}; refined "
+              f"{r['refined']['speedup']:.2f}x, gain loss {r['refined']['mean_gain_loss']:.4f}",
               file=sys.stderr, flush=True)
     except Exception as e:
         res["proposal"] = {"error": repr(e)}
