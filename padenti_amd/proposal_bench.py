@@ -142,16 +142,18 @@ def run(a):
     screened_tree, t_screen, st_screen = timed_train(tr, ts, pool, a.depth, pa.PD_MODE_PROPOSAL)
     # refinement: the network's best candidates ranked exactly on a subsample;
     # beside it the same refinement of the whole pool (no use of the network)
-    rscreen = a.refine_screen or a.pool
+    # the same ranked nodes as the plain proposal (a ranked node below
+    # --refine-min samples is ranked on all of them)
+    rscreen, rmin = a.refine_screen or a.pool, a.min_samples
     tr.set_proposal_refine(a.refine_stride, a.refine_min)
-    tr.set_proposal(net, keep=a.keep, min_samples=a.min_samples, judge=False, screen=rscreen)
+    tr.set_proposal(net, keep=a.keep, min_samples=rmin, judge=False, screen=rscreen)
     ref_tree, t_ref, st_ref = timed_train(tr, ts, pool, a.depth, pa.PD_MODE_PROPOSAL)
-    tr.set_proposal(net, keep=a.keep, min_samples=a.min_samples, judge=True, screen=rscreen)
+    tr.set_proposal(net, keep=a.keep, min_samples=rmin, judge=True, screen=rscreen)
     _, _, st_ref_j = timed_train(tr, ts, pool, a.depth, pa.PD_MODE_PROPOSAL)
     if rscreen < a.pool:
-        tr.set_proposal(net, keep=a.keep, min_samples=a.min_samples, judge=False, screen=a.pool)
+        tr.set_proposal(net, keep=a.keep, min_samples=rmin, judge=False, screen=a.pool)
         refall_tree, t_refall, _ = timed_train(tr, ts, pool, a.depth, pa.PD_MODE_PROPOSAL)
-        tr.set_proposal(net, keep=a.keep, min_samples=a.min_samples, judge=True, screen=a.pool)
+        tr.set_proposal(net, keep=a.keep, min_samples=rmin, judge=True, screen=a.pool)
         _, _, st_refall_j = timed_train(tr, ts, pool, a.depth, pa.PD_MODE_PROPOSAL)
     else:
         refall_tree, t_refall, st_refall_j = ref_tree, t_ref, st_ref_j
@@ -203,7 +205,7 @@ def run(a):
         "mean_gain_loss": gl(st_judge),
         "mean_gain_loss_screened": gl(st_screen), "screen": a.screen,
         "mean_gain_loss_no_network": gl(st_const),
-        "refined": {"screen": rscreen, "stride": a.refine_stride, "min_samples": a.refine_min,
+        "refined": {"screen": rscreen, "stride": a.refine_stride, "min_samples": a.refine_min, "min_ranked": rmin,
                     "tree_s": t_ref, "speedup": t_exh / t_ref, "mean_gain_loss": gl(st_ref_j),
                     "accuracy_loss_pt": 100 * (acc["exhaustive"] - acc["refined"]),
                     "whole_pool": {"tree_s": t_refall, "speedup": t_exh / t_refall,
