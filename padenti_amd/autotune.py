@@ -20,9 +20,12 @@ split device time, pd_trainer_time_level).  The entry keeps the tuning with
 the smallest total and, per shape, the fastest one as a per-level rule: a
 level whose active samples per trainable node are nearest (geometrically) to
 a shape's uses that shape's tuning.  Real frontiers are uneven (a few large
-nodes, many small ones), so the choice is then checked end to end: one real
-tree is trained with the default, the best single tuning and the per-level
-rules, and the fastest of the three is stored.  Entries are keyed by (feature hook,
+nodes, many small ones), so a second set of rules comes from real trees: the
+six best tunings and the default each train one tree, and every level takes
+its fastest tuning (by its samples per node).  The choice is then checked end
+to end: one real tree is trained with the default, the best single tuning,
+the shape rules and the real-tree rules, and the simplest of them within 2%
+of the fastest is stored.  Entries are keyed by (feature hook,
 threshold bucket, classes); HIPTreeTrainer applies the matching entry before
 every train() call.
 """
@@ -85,6 +88,48 @@ def rules_from_sweep(res, shapes, S):
     return rules
 
 
+def rules_from_levels(levels, base):
+    """Per-level rules from real-tree levels: `levels` = [(samples per node,
+    {tuning: ms})]; each level takes its fastest tuning unless it is within
+    4% of `base`'s time there (run-to-run noise of one level is a few
+    percent); runs of levels (by decreasing samples per node)
+    with the same tuning become one rule, switching at the geometric mean of
+    the neighbouring levels' samples per node; the last run is the base."""
+    order = sorted(levels, key=lambda l: -l[0])
+    pick = []
+    for spn, ms in order:
+        best = min(ms, key=ms.get)
+        pick.append((spn, best if ms[best] < 0.96 * ms[base] else base))
+    runs = []
+    for spn, t in pick:
+        if runs and runs[-1][2] == t:
+            runs[-1][1] = spn
+        else:
+            runs.append([spn, spn, t])
+    base_t = runs[-1][2] if runs else base
+    rules = []
+    for i in range(len(runs) - 1):
+        m = int((runs[i][1] * runs[i + 1][0]) ** 0.5)
+        rules.append((max(m, 1), runs[i][2][0], runs[i][2][1]))
+    # rules must have strictly decreasing thresholds
+    out = []
+    for r in rules:
+        if not out or r[0] < out[-1][0]:
+            out.append(r)
+    return base_t, out
+
+
+def tree_levels(tr, ts, params, c, tuning):
+    """[(samples per node, hist + split ms)] of every level of one real tree."""
+    import padenti_amd as pa
+    set_tuning(tr, *tuning)
+    tree = pa.Tree(0, c["depth"], c["kind"], c["n_classes"])
+    tr.train(tree, ts, params, 1, c["depth"])
+    tr.train(tree, ts, params, 1, c["depth"])
+    return [(s["active_samples"] / max(1, s["frontier_nodes"]), s["hist_ms"] + s["split_ms"])
+            for s in tr.last_level_stats]
+
+
 def sweep(trainer, params, shapes=SHAPES, reps=2, log=None):
     """Device ms of every (waves, blocks_per_cu) on every shape."""
     out = {}
@@ -125,26 +170,49 @@ def tune_workload(cfg_name, n_images=None, reps=2, log=None):
     # shares): the default, the best single tuning and the per-level rules;
     # the fastest is kept.
     def tree_s(w, b, r):
+        # the smaller of two timed trees after a warm-up tree
         set_tuning(tr, w, b, r)
         tree = pa.Tree(0, c["depth"], c["kind"], c["n_classes"])
         tr.train(tree, ts, p, 1, c["depth"])
-        pa.lib().pd_dev_sync(0)
-        t = time.perf_counter()
-        tr.train(tree, ts, p, 1, c["depth"])
-        pa.lib().pd_dev_sync(0)
-        return time.perf_counter() - t
+        ts_ = []
+        for _ in range(2):
+            pa.lib().pd_dev_sync(0)
+            t = time.perf_counter()
+            tr.train(tree, ts, p, 1, c["depth"])
+            pa.lib().pd_dev_sync(0)
+            ts_.append(time.perf_counter() - t)
+        return min(ts_)
+
+    # per-level rules measured on real tree levels: the six best tunings of
+    # the sweep and the default, one tree each
+    cands = sorted(total, key=total.get)[:6]
+    if (8, 16) not in cands:
+        cands.append((8, 16))
+    lv = {t: tree_levels(tr, ts, p, c, t) for t in cands}
+    levels = [(lv[best][i][0], {t: lv[t][i][1] for t in cands}) for i in range(len(lv[best]))]
+    tbase, trules = rules_from_levels(levels, best)
+    if log:
+        log(f"[autotune] {cfg_name} real-tree levels: base {tbase}, rules {trules}")
 
     t_default = tree_s(8, 16, ())
     t_single = tree_s(best[0], best[1], ())
     t_rules = tree_s(best[0], best[1], rules)
+    t_trules = tree_s(tbase[0], tbase[1], trules)
     if log:
         log(f"[autotune] {cfg_name} tree: default {t_default:.3f} s, best single {t_single:.3f} s, "
-            f"per-level rules {t_rules:.3f} s")
-    choice = min((t_default, "default"), (t_single, "single"), (t_rules, "rules"))[1]
+            f"per-level rules {t_rules:.3f} s, real-tree rules {t_trules:.3f} s")
+    # the simplest of the four within 2% of the fastest (a tree's time varies
+    # by about that much from run to run)
+    cand_t = [(t_default, "default"), (t_single, "single"), (t_rules, "rules"), (t_trules, "tree_rules")]
+    fastest = min(t for t, _ in cand_t)
+    choice = next(name for t, name in cand_t if t <= 1.02 * fastest)
+    shape_rules = rules
     if choice == "default":
         best, rules = (8, 16), []
     elif choice == "single":
         rules = []
+    elif choice == "tree_rules":
+        best, rules = tbase, trules
     tr.close()
     return {
         "key": key(c["kind"], p.n_thresholds, c["n_classes"]),
@@ -155,7 +223,13 @@ def tune_workload(cfg_name, n_images=None, reps=2, log=None):
         "total_ms": total[best], "per_level_total_ms": per_shape, "default_total_ms": default,
         "gain_vs_default": default / total[best], "per_level_gain_vs_default": default / per_shape,
         "shapes": list(shapes),
-        "tree_s": {"default": t_default, "single": t_single, "rules": t_rules, "chosen": choice},
+        "tree_s": {"default": t_default, "single": t_single, "rules": t_rules, "tree_rules": t_trules,
+                   "chosen": choice},
+        "shape_rules": [{"min_samples_per_node": m, "waves": w, "blocks_per_cu": b} for m, w, b in shape_rules],
+        "tree_rule_base": {"waves": tbase[0], "blocks_per_cu": tbase[1]},
+        "tree_rules": [{"min_samples_per_node": m, "waves": w, "blocks_per_cu": b} for m, w, b in trules],
+        "tree_levels": [{"waves": t[0], "blocks_per_cu": t[1], "levels": [[round(a, 1), b] for a, b in lv[t]]}
+                        for t in cands],
         "sweep": [{"waves": k[0], "blocks_per_cu": k[1], "ms": [v[n] for n in shapes]} for k, v in res.items()],
         "sweep_s": time.perf_counter() - t0,
     }

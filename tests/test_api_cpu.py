@@ -81,3 +81,14 @@ def test_autotune_rules_from_sweep():
     rules = autotune.rules_from_sweep(res, shapes, 4096)
     # 16 waves for 4096 and 256 samples per node, 8 waves below sqrt(256 * 16) = 64
     assert rules == [(64, 16, 64), (0, 8, 16)]
+
+
+def test_autotune_rules_from_tree_levels():
+    from padenti_amd import autotune
+    a, b = (8, 16), (16, 64)
+    levels = [(1e7, {a: 141.0, b: 155.0}), (5e6, {a: 142.0, b: 159.0}), (12400, {a: 173.0, b: 189.0}),
+              (7190, {a: 186.0, b: 177.0}), (1000, {a: 246.0, b: 228.0}), (500, {a: 250.0, b: 249.0})]
+    # deep levels (below sqrt(12400 * 7190)) take 16 waves, which becomes the base; within 4% keeps the run
+    assert autotune.rules_from_levels(levels, b) == (b, [(int((12400 * 7190) ** 0.5), 8, 16)])
+    # one tuning everywhere: no rules
+    assert autotune.rules_from_levels([(1e6, {a: 1.0, b: 2.0}), (10, {a: 1.0, b: 2.0})], a) == (a, [])
