@@ -157,6 +157,28 @@ __device__ __forceinline__ void hist_add(typename F::FeatType v, uint4 r, const 
   if (fv) atomicAdd(&bins[crow + (rk >> 1) * 64u + lane], 1u << ((rk & 1u) * 16u));
 }
 
+// Class fraction p / d of a split side (counts, p <= d): the plain IEEE
+// division, or (FAST, d < 2^21) the product with the side's correctly
+// rounded reciprocal y = RN(1/d) and one FMA residual correction, which is
+// already RN(p/d): with |p/d| in [2^e, 2^(e+1)) the corrected value is within
+// 2^(e-46) of p/d, while p - m*d is a non-zero multiple of 2^(e-25) for every
+// rounding midpoint m (p an integer, m an odd multiple of 2^(e-25) with a
+// 25-bit numerator), so |p/d - m| >= 2^(e-25)/d > 2^(e-46).
+constexpr int COUNT_RATIO_FAST_MAX = 1 << 21;
+template <bool FAST>
+__device__ __forceinline__ float count_ratio(uint32_t p, float fd, float y)
+{
+  const float fp = (float)p;
+  if constexpr (FAST) {
+    const float q = fp * y;
+    const float r = __builtin_fmaf(-fd, q, fp);
+    return __builtin_fmaf(r, y, q);
+  } else {
+    (void)y;
+    return fp / fd;
+  }
+}
+
 // Per-slot mode bits of the histogram kernel.
 constexpr uint8_t SLOT_ATOMIC = 1;   // several chunks add into H (else plain stores)
 constexpr uint8_t SLOT_FUSED = 2;    // one chunk: evaluate the gains in-kernel, no H
@@ -400,25 +422,30 @@ __global__ __launch_bounds__(NWAVES * 64) void k_hist(
   const float h = slot_h[it.slot];
   unsigned long long best = 0ull;
   if (fv) {
-    for (uint32_t t = wave; t < nT; t += NW) {
-      const uint32_t l = sl[t * 64 + lane];
-      int nL = 0;
-      for (uint32_t c = 0; c < nC; c++) nL += (int)prefix_at(bins, RW, c, l, lane);
-      const int nR = N - nL;
-      const float fL = (float)nL, fR = (float)nR;
-      float hL = 0.0f, hR = 0.0f;
-      for (uint32_t c = 0; c < nC; c++) {
-        const uint32_t lc = prefix_at(bins, RW, c, l, lane);
-        float tmp = nL ? (float)lc / fL : 0.0f;
-        hL -= plogp(tmp);
-        tmp = nR ? (float)(tt[c] - lc) / fR : 0.0f;
-        hR -= plogp(tmp);
+    auto scan = [&](auto fast) {
+      for (uint32_t t = wave; t < nT; t += NW) {
+        const uint32_t l = sl[t * 64 + lane];
+        int nL = 0;
+        for (uint32_t c = 0; c < nC; c++) nL += (int)prefix_at(bins, RW, c, l, lane);
+        const int nR = N - nL;
+        const float fL = (float)nL, fR = (float)nR;
+        const float yL = 1.0f / fL, yR = 1.0f / fR;   // used by the fast form only
+        float hL = 0.0f, hR = 0.0f;
+        for (uint32_t c = 0; c < nC; c++) {
+          const uint32_t lc = prefix_at(bins, RW, c, l, lane);
+          float tmp = nL ? count_ratio<decltype(fast)::value>(lc, fL, yL) : 0.0f;
+          hL -= plogp(tmp);
+          tmp = nR ? count_ratio<decltype(fast)::value>(tt[c] - lc, fR, yR) : 0.0f;
+          hR -= plogp(tmp);
+        }
+        const float a = fL / fN * hL;
+        const float b = fR / fN * hR;
+        const unsigned long long key = split_key(h - (a + b), t * nF + f);
+        best = key > best ? key : best;
       }
-      const float a = fL / fN * hL;
-      const float b = fR / fN * hR;
-      const unsigned long long key = split_key(h - (a + b), t * nF + f);
-      best = key > best ? key : best;
-    }
+    };
+    if (N < COUNT_RATIO_FAST_MAX) scan(std::true_type{});
+    else scan(std::false_type{});
   }
   best = wave_max_u64(best);
   if (lane == 0) red[wave] = best;
@@ -559,7 +586,7 @@ __device__ __forceinline__ float node_entropy(const uint32_t *tot, uint32_t nC, 
 
 // Gain of one pair given left counts; op order of learn_best_feature.cl:63-93
 // (h is the node entropy, the same value the kernel recomputes per pair).
-template <int NCM>
+template <int NCM, bool FAST>
 __device__ __forceinline__ float pair_gain(const uint32_t *left, const uint32_t *tot, uint32_t nC, float h, int N)
 {
   int nL = 0;
@@ -568,13 +595,14 @@ __device__ __forceinline__ float pair_gain(const uint32_t *left, const uint32_t 
     if ((uint32_t)l < nC) nL += (int)left[l];
   const int nR = N - nL;
   const float fN = (float)N, fL = (float)nL, fR = (float)nR;
+  const float yL = 1.0f / fL, yR = 1.0f / fR;   // used by the fast form only
   float hL = 0.0f, hR = 0.0f;
 #pragma unroll
   for (int l = 0; l < NCM; l++) {
     if ((uint32_t)l < nC) {
-      float tmp = nL ? (float)left[l] / fL : 0.0f;
+      float tmp = nL ? count_ratio<FAST>(left[l], fL, yL) : 0.0f;
       hL -= plogp(tmp);
-      tmp = nR ? (float)(tot[l] - left[l]) / fR : 0.0f;
+      tmp = nR ? count_ratio<FAST>(tot[l] - left[l], fR, yR) : 0.0f;
       hR -= plogp(tmp);
     }
   }
@@ -609,15 +637,19 @@ __global__ __launch_bounds__(SPLIT_THREADS) void k_split(const uint32_t *__restr
   const uint32_t k0 = part * pairs_per_block;
   const uint32_t k1 = min(npairs, k0 + pairs_per_block);
   uint64_t key = 0;
-  for (uint32_t k = k0 + threadIdx.x; k < k1; k += SPLIT_THREADS) {
-    uint32_t left[NCM];
+  auto scan = [&](auto fast) {
+    for (uint32_t k = k0 + threadIdx.x; k < k1; k += SPLIT_THREADS) {
+      uint32_t left[NCM];
 #pragma unroll
-    for (int l = 0; l < NCM; l++) left[l] = ((uint32_t)l < nC) ? Hs[(size_t)l * npairs + k] : 0u;
-    const float g = pair_gain<NCM>(left, tv, nC, h, N);
-    if (all_gains) all_gains[(size_t)slot * npairs + k] = g;
-    const uint64_t kk = split_key(g, k);
-    key = kk > key ? kk : key;
-  }
+      for (int l = 0; l < NCM; l++) left[l] = ((uint32_t)l < nC) ? Hs[(size_t)l * npairs + k] : 0u;
+      const float g = pair_gain<NCM, decltype(fast)::value>(left, tv, nC, h, N);
+      if (all_gains) all_gains[(size_t)slot * npairs + k] = g;
+      const uint64_t kk = split_key(g, k);
+      key = kk > key ? kk : key;
+    }
+  };
+  if (N < COUNT_RATIO_FAST_MAX) scan(std::true_type{});
+  else scan(std::false_type{});
   key = wave_max_u64(key);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = key;
   __syncthreads();

@@ -198,8 +198,9 @@ PD_API int pd_md5rand(int device, uint32_t n, const uint32_t *seeds, uint32_t *o
 PD_API int pd_log2_det(int device, uint32_t n, const float *x, float *y);
 /* q[i] = p[i] / d[i] through the feature hooks' per-pixel division path. */
 PD_API int pd_fast_div(int device, uint32_t n, const float *p, const float *d, float *q);
-/* q[i] = p[i] / d[i] through the integer-depth hooks' division (d[i] an
- * integer, 1 <= |d[i]| <= 65535). */
+/* q[i] = p[i] / d[i] through the one-correction quotient of the integer-depth
+ * hooks (d[i] an integer, 1 <= |d[i]| <= 65535, any p[i]) and of the split
+ * search's class fractions (p[i] <= d[i] < 2^21, both integers). */
 PD_API int pd_int_div(int device, uint32_t n, const float *p, const float *d, float *q);
 PD_API int pd_feature_eval(int device, int feature_kind, const void *image, uint32_t w, uint32_t h, uint32_t n,
                     const int32_t *xy, const void *params, void *out);

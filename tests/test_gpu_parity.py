@@ -106,6 +106,31 @@ def test_int_division_device(gpu):
     assert np.all(np.abs(q[~normal]) < 0.5)
 
 
+def test_count_ratio_division_device(gpu):
+    """Class fractions p / d of a split (integer counts, p <= d < 2^21) through
+    the one-correction quotient equal IEEE float32 division: random pairs
+    over the whole range, every d up to 4096 with all p, p in {1, d-1, d}
+    and numerators next to the midpoints of half-integer-free quotients."""
+    rng = np.random.default_rng(13)
+    ds, ps = [], []
+    d = rng.integers(1, 1 << 21, 3_000_000)
+    ds.append(d)
+    ps.append((rng.random(d.size) * (d + 1)).astype(np.int64).clip(0, d))
+    for k in range(1, 4097):
+        ds.append(np.full(k + 1, k))
+        ps.append(np.arange(k + 1))
+    d2 = rng.integers(2, 1 << 21, 500_000)
+    for pp in (np.ones_like(d2), d2 - 1, d2):
+        ds.append(d2)
+        ps.append(pp)
+    d = np.concatenate(ds).astype(np.float32)
+    p = np.concatenate(ps).astype(np.float32)
+    assert np.all(d < 2 ** 21) and np.all(p <= d)
+    q = np.zeros_like(p)
+    L.check(L.lib().pd_int_div(0, p.size, L.ptr(p), L.ptr(d), L.ptr(q)))
+    assert np.array_equal(q, (p / d).astype(np.float32))
+
+
 def _rand_image(kind, rng, w, h):
     if kind == pa.PD_FEAT_DEPTH_F32:
         im = rng.uniform(0.5, 6.0, (h, w)).astype(np.float32)
