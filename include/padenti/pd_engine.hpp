@@ -749,8 +749,21 @@ struct Trainer final : TrainerBase {
   // Split search + finalize of a slice of ns slots whose H blocks are
   // hslots (the fused slots were searched by the histogram kernel);
   // res_out[q*(nC+2) + {0: k, 1: gain bits, 2..: left counts}].
+  // winners' parameters and thresholds of the slots of a level, filled by
+  // split_slice through evaluate when set (train() sets them per level)
+  std::vector<T> *lvl_par = nullptr, *lvl_thr = nullptr;
+  DevBuf d_wpar, d_wthr;
+
+  // With `win` (the slice's candidate map, tree and bounds) the winners'
+  // parameters and thresholds come back too (wpar [ns][FD], wthr [ns]).
+  struct WinnerArgs {
+    const uint32_t *d_map;
+    uint32_t tree;
+    const Bounds<F> *bnd;
+    std::vector<T> *wpar, *wthr;
+  };
   void split_slice(uint32_t ns, const std::vector<uint32_t> &hslots, uint32_t nT, uint32_t nF, int mode,
-                   std::vector<uint32_t> &res_out, float *all_gains = nullptr)
+                   std::vector<uint32_t> &res_out, float *all_gains = nullptr, const WinnerArgs *win = nullptr)
   {
     const uint32_t nh = (uint32_t)hslots.size();
     d_best.ensure((size_t)ns * sizeof(uint64_t));
@@ -772,7 +785,28 @@ struct Trainer final : TrainerBase {
                        d_res.as<uint32_t>());
     HIPCHK(hipGetLastError());
     res_out.resize((size_t)ns * (nC + 2));
-    d2h_sync(res_out.data(), d_res.p, res_out.size() * sizeof(uint32_t));
+    if (!win) {
+      d2h_sync(res_out.data(), d_res.p, res_out.size() * sizeof(uint32_t));
+      return;
+    }
+    d_wpar.ensure((size_t)ns * FD * sizeof(T));
+    d_wthr.ensure((size_t)ns * sizeof(T));
+    hipLaunchKernelGGL((k_winner_params<F>), dim3((ns + 255) / 256), dim3(256), 0, stream, win->tree,
+                       d_slot_node.as<int32_t>(), d_res.as<uint32_t>(), ns, nC, nF, win->d_map, *win->bnd,
+                       d_wpar.as<T>(), d_wthr.as<T>());
+    HIPCHK(hipGetLastError());
+    win->wpar->resize((size_t)ns * FD);
+    win->wthr->resize(ns);
+    const size_t br = res_out.size() * sizeof(uint32_t), bp = (size_t)ns * FD * sizeof(T), bt = ns * sizeof(T);
+    void *pr = stage.alloc(br), *pp = stage.alloc(bp), *pt = stage.alloc(bt);
+    HIPCHK(hipMemcpyAsync(pr, d_res.p, br, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(pp, d_wpar.p, bp, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(pt, d_wthr.p, bt, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    std::copy((const char *)pr, (const char *)pr + br, (char *)res_out.data());
+    std::copy((const char *)pp, (const char *)pp + bp, (char *)win->wpar->data());
+    std::copy((const char *)pt, (const char *)pt + bt, (char *)win->wthr->data());
+    stage.reset();
   }
 
   // network input of a node: class histogram n_c / n (16) and depth / 16
@@ -893,7 +927,9 @@ struct Trainer final : TrainerBase {
         d_gains.ensure((size_t)ns * nT * nFe * sizeof(float));
         gains = d_gains.as<float>();
       }
-      split_slice(ns, hslots, nT, nFe, mode, res, gains);
+      std::vector<T> wpar, wthr;
+      const WinnerArgs win{d_map_all ? d_map_all + s0 * nFe : nullptr, tree, &bnd, &wpar, &wthr};
+      split_slice(ns, hslots, nT, nFe, mode, res, gains, lvl_par ? &win : nullptr);
       HIPCHK(hipEventRecord(e2.e, stream));
       HIPCHK(hipEventSynchronize(e2.e));
       stage.reset();
@@ -907,6 +943,10 @@ struct Trainer final : TrainerBase {
         uint32_t *o = &res_all[(size_t)idx[s0 + q] * (nC + 2)];
         o[0] = t * nFp + fs;
         std::copy(r + 1, r + nC + 2, o + 1);
+        if (lvl_par) {
+          std::copy(&wpar[(size_t)q * FD], &wpar[(size_t)q * FD] + FD, &(*lvl_par)[(size_t)idx[s0 + q] * FD]);
+          (*lvl_thr)[idx[s0 + q]] = wthr[q];
+        }
       }
       s0 = s1;
     }
@@ -1072,8 +1112,12 @@ struct Trainer final : TrainerBase {
       if (prop_judge) {
         std::vector<uint32_t> res_j(slots.size() * (nC + 2), 0u);
         pd_level_stats js{};
+        std::vector<T> *keep_par = lvl_par, *keep_thr = lvl_thr;
+        lvl_par = lvl_thr = nullptr;   // the judge's winners are not the tree's
         evaluate(slots, gidx, act, P, K, nullptr, nullptr, bnd, tree, PD_MODE_LEVEL_BATCHED,
                  pick_chunk(gact, (K + 63) / 64), classTot, level, false, res_j, js);
+        lvl_par = keep_par;
+        lvl_thr = keep_thr;
         st.judge_ms += js.hist_ms + js.split_ms;
         for (uint32_t q : gidx) {
           float ge, gp;
@@ -1182,9 +1226,19 @@ struct Trainer final : TrainerBase {
         (pr ? prop_idx : plain_idx).push_back(q);
       }
       const int lmode = mode == PD_MODE_PER_NODE ? PD_MODE_PER_NODE : PD_MODE_LEVEL_BATCHED;
-      evaluate(slots, plain_idx, act, P, nF, nullptr, nullptr, bnd, tree_id, lmode, ch, classTot, d,
-               sink_x != nullptr && mode != PD_MODE_PROPOSAL, res_all, st);
-      if (!prop_idx.empty()) propose(slots, prop_idx, act, P, bnd, tree_id, classTot, d, res_all, st);
+      // the winners' parameters and thresholds, re-derived on the device
+      std::vector<T> win_par(slots.size() * FD), win_thr(slots.size());
+      {
+        struct Reset {   // the pointers never outlive this level, exceptions included
+          Trainer *tr;
+          ~Reset() { tr->lvl_par = tr->lvl_thr = nullptr; }
+        } reset{this};
+        lvl_par = &win_par;
+        lvl_thr = &win_thr;
+        evaluate(slots, plain_idx, act, P, nF, nullptr, nullptr, bnd, tree_id, lmode, ch, classTot, d,
+                 sink_x != nullptr && mode != PD_MODE_PROPOSAL, res_all, st);
+        if (!prop_idx.empty()) propose(slots, prop_idx, act, P, bnd, tree_id, classTot, d, res_all, st);
+      }
       if (mode == PD_MODE_PROPOSAL) {
         for (uint32_t q : plain_idx)
           if (slots[q].trainable) st.fallback_nodes++;
@@ -1208,7 +1262,6 @@ struct Trainer final : TrainerBase {
           rS += tt[c] - lh[c];
         }
         if (!lS || !rS) continue;
-        const uint32_t bt = k / nF, bf = k % nF;
         const size_t L = 2 * (size_t)n + 1, R = L + 1;
         nodeTot[L] = lS;
         nodeTot[R] = rS;
@@ -1220,8 +1273,8 @@ struct Trainer final : TrainerBase {
         }
         left[L] = -1;
         left[R] = -1;
-        candidate_params<T, FD>(tree_id, n, bf, bnd.lo, bnd.hi, &feats[(size_t)n * FD]);
-        thrs[n] = candidate_threshold<T>(tree_id, n, bf, bt, bnd.tlo, bnd.thi);
+        std::copy(&win_par[(size_t)q * FD], &win_par[(size_t)q * FD] + FD, &feats[(size_t)n * FD]);
+        thrs[n] = win_thr[q];
         left[n] = (int32_t)L;
         split_flag[q] = 1;
         st.split_nodes++;

@@ -967,6 +967,30 @@ __global__ __launch_bounds__(256) void k_fill_i32(int32_t *__restrict__ p, uint6
   for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256u) p[i] = v;
 }
 
+// Parameters and threshold of each slot's winning pair (res rows
+// {k = t*nFe + f, ...}; f mapped through cand_map when given), re-derived
+// from the sampler for the tree arrays (the reference re-derives them on
+// the host, cl_tree_trainer_impl_learn_best_featthr.hpp:183-222).
+template <class F>
+__global__ __launch_bounds__(256) void k_winner_params(uint32_t tree, const int32_t *__restrict__ slot_node,
+                                                       const uint32_t *__restrict__ res, uint32_t ns, uint32_t nC,
+                                                       uint32_t nFe, const uint32_t *__restrict__ cand_map,
+                                                       Bounds<F> bnd, typename F::FeatType *__restrict__ par,
+                                                       typename F::FeatType *__restrict__ thr)
+{
+  using T = typename F::FeatType;
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= ns) return;
+  const uint32_t k = res[(size_t)q * (nC + 2)];
+  const uint32_t t = k / nFe, f = k % nFe;
+  const uint32_t fs = cand_map ? cand_map[(size_t)q * nFe + f] : f;
+  const uint32_t node = (uint32_t)slot_node[q];
+  T p[F::FeatDim];
+  candidate_params<T, F::FeatDim>(tree, node, fs, bnd.lo, bnd.hi, p);
+  for (int j = 0; j < F::FeatDim; j++) par[(size_t)q * F::FeatDim + j] = p[j];
+  thr[q] = candidate_threshold<T>(tree, node, fs, t, bnd.tlo, bnd.thi);
+}
+
 // Evenly spaced subsample of every slot's records (the refinement stage of
 // the split proposal): slot q's dst_count[q] records at dst_begin[q] are
 // src[src_begin[q] + floor(j * src_count[q] / dst_count[q])], j < dst_count[q]
