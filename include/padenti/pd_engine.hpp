@@ -1197,6 +1197,30 @@ struct Trainer final : TrainerBase {
 
     std::vector<SlotInfo> slots{SlotInfo{0, 0u, (uint32_t)S, S > thr}};
     Event e3, e4;
+    // The routed child populations of a level come back with the next
+    // synchronization (the next level's split search, or the end of
+    // training): the host goes on queueing the next level behind the
+    // routing instead of waiting for it.  Every split node's children must
+    // hold the populations of its chosen split.
+    struct RouteCheck {
+      bool pending = false;
+      uint32_t level = 0;
+      const uint32_t *counts = nullptr;   // pinned [2][nq], filled by an async copy
+      std::vector<std::pair<int32_t, std::pair<uint32_t, uint32_t>>> want;   // (node, (left, right)) per slot
+    } rc;
+    PinnedStage route_stage;
+    auto check_route = [&]() {
+      if (!rc.pending) return;
+      rc.pending = false;
+      for (size_t q = 0; q < rc.want.size(); q++) {
+        if (rc.want[q].first < 0) continue;
+        if (rc.counts[2 * q] != rc.want[q].second.first || rc.counts[2 * q + 1] != rc.want[q].second.second)
+          throw ParityError("routing of node " + std::to_string(rc.want[q].first) +
+                            " disagrees with its histogram split");
+      }
+      if (ls) ls[rc.level - 1].route_ms = elapsed_ms(e3, e4);
+      route_stage.reset();
+    };
 
     for (uint32_t d = 1; d < end_depth; d++) {
       const auto t_start = std::chrono::steady_clock::now();
@@ -1239,6 +1263,7 @@ struct Trainer final : TrainerBase {
                  sink_x != nullptr && mode != PD_MODE_PROPOSAL, res_all, st);
         if (!prop_idx.empty()) propose(slots, prop_idx, act, P, bnd, tree_id, classTot, d, res_all, st);
       }
+      check_route();   // the split search synchronized the stream
       if (mode == PD_MODE_PROPOSAL) {
         for (uint32_t q : plain_idx)
           if (slots[q].trainable) st.fallback_nodes++;
@@ -1361,15 +1386,18 @@ struct Trainer final : TrainerBase {
                          d_sbegin.as<uint32_t>(), nq, d_ccount.as<uint32_t>());
       HIPCHK(hipGetLastError());
       HIPCHK(hipEventRecord(e4.e, stream));
-      std::vector<uint32_t> ccount(2 * (size_t)nq);
-      d2h_sync(ccount.data(), d_ccount.p, ccount.size() * sizeof(uint32_t));
-      st.route_ms = elapsed_ms(e3, e4);
-      // the routed child populations must equal the split's class counts
-      for (size_t q = 0; q < slots.size(); q++) {
-        if (!split_flag[q]) continue;
-        const int32_t n = slots[q].node;
-        if (ccount[2 * q] != nodeTot[2 * n + 1] || ccount[2 * q + 1] != nodeTot[2 * n + 2])
-          throw ParityError("routing of node " + std::to_string(n) + " disagrees with its histogram split");
+      {
+        uint32_t *cc = (uint32_t *)route_stage.alloc(2 * (size_t)nq * sizeof(uint32_t));
+        HIPCHK(hipMemcpyAsync(cc, d_ccount.p, 2 * (size_t)nq * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        rc.pending = true;
+        rc.level = d;
+        rc.counts = cc;
+        rc.want.assign(nq, {-1, {0u, 0u}});
+        for (size_t q = 0; q < slots.size(); q++) {
+          if (!split_flag[q]) continue;
+          const int32_t n = slots[q].node;
+          rc.want[q] = {n, {nodeTot[2 * n + 1], nodeTot[2 * n + 2]}};
+        }
       }
       std::swap(act, act_next);
       std::swap(aslot, aslot_next);
@@ -1381,6 +1409,10 @@ struct Trainer final : TrainerBase {
       slots.swap(next);
       st.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
       if (ls) ls[d - 1] = st;
+    }
+    if (rc.pending) {
+      HIPCHK(hipStreamSynchronize(stream));
+      check_route();
     }
   }
 
