@@ -332,7 +332,7 @@ struct TrainerBase {
   uint32_t nC = 0;
   uint64_t hist_budget = 8ull << 30;
   uint32_t chunk = 0;
-  bool probe_order = true;   // probe-coherent candidate tiles (functors with kProbeParam)
+  bool probe_order = true;   // probe-coherent candidate tiles (functors with kProbeRowA/B)
   pd_tuning tune{8u, 16u};   // waves per histogram block, target blocks per CU
   std::vector<pd_tuning_rule> tune_rules;   // per level, by samples per node (decreasing)
   pd_tuning level_tuning(uint64_t active, uint32_t trainable) const
@@ -690,7 +690,7 @@ struct Trainer final : TrainerBase {
     }
     if (ev_begin) HIPCHK(hipEventRecord(ev_begin->e, stream));
     d_perm_cur = nullptr;
-    if constexpr (probe_param<F>::value >= 0) {
+    if constexpr (probe_rows<F>::on) {
       if (probe_order && nF > 64 && nF <= ORDER_MAX) {
         d_perm.ensure((size_t)ns * nF * sizeof(uint16_t));
         hipLaunchKernelGGL((k_cand_order<F>), dim3(ns), dim3(ORDER_THREADS), 0, stream, tree,

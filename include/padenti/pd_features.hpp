@@ -205,7 +205,7 @@ struct DepthU16Feature {
   static constexpr int nChannels = 1;
   static constexpr int FeatDim = 4;
   static constexpr int kind = 1;
-  static constexpr int kProbeParam = 0;   // p[0], p[1]: offset of the first probe
+  static constexpr int kProbeRowA = 1, kProbeRowB = 3;   // row offsets of the two probes
   PD_HD static Prep prepare(const ImageView<uint16_t, 1> &im, int x, int y)
   {
     return int_div_prep((float)im.at(x, y, 0));
@@ -234,7 +234,7 @@ struct RGBDFeature {
   static constexpr int nChannels = 4;
   static constexpr int FeatDim = 6;
   static constexpr int kind = 2;
-  static constexpr int kProbeParam = 0;
+  static constexpr int kProbeRowA = 1, kProbeRowB = 3;
   PD_HD static int chan(float c)
   {
     if (!(c > 0.0f)) return 0;
@@ -269,7 +269,7 @@ struct DemoS16Feature {
   static constexpr int nChannels = 1;
   static constexpr int FeatDim = 2;
   static constexpr int kind = 3;
-  static constexpr int kProbeParam = 0;
+  static constexpr int kProbeRowA = 1, kProbeRowB = 1;   // one probe, row offset p[1]
   PD_HD static Prep prepare(const ImageView<uint16_t, 1> &im, int x, int y)
   {
     return int_div_prep((float)(int16_t)im.at(x, y, 0));

@@ -9,8 +9,9 @@
 // Histogram design (replaces hist_update.cl:36-187 + the host SSE2
 // accumulation cl_tree_trainer_impl_traverse_ts.hpp:518-617):
 //  * one work item = (frontier slot, chunk of that node's samples, tile of 64
-//    candidate features); a block of 4 wave64s owns the item;
-//  * lane l of every wave owns candidate f = 64*tile + l: its FeatDim
+//    candidate features); a block of 4, 8 or 16 wave64s owns the item;
+//  * lane l of every wave owns candidate f = perm[64*tile + l] (the node's
+//    probe-coherent order, or f = 64*tile + l without one): its FeatDim
 //    parameters and nT thresholds are derived by MD5 once per block (as the
 //    reference derives them per (sample, feature)) and then held in VGPRs;
 //  * the waves stride over the chunk's samples; a sample is wave-uniform, so
@@ -193,40 +194,67 @@ __device__ __forceinline__ uint32_t prefix_at(const uint32_t *bins, int RW, uint
 
 // ---------------------------------------------------------------------------
 // Probe-coherent candidate order.  A functor may declare
-//   static constexpr int kProbeParam = j;
-// when p[j], p[j+1] are the x / y offset numerators of its first texel probe.
-// The trainer then lays each node's candidates out along a Morton curve of
-// (p[j], p[j+1]) before cutting them into 64-lane tiles: the first probes of
-// the lanes of one gather land close together and touch far fewer cache
-// lines.  The order only decides which lane evaluates which candidate; every
-// count, key and output column stays that of the candidate's own index.
+//   static constexpr int kProbeRowA = a, kProbeRowB = b;
+// when p[a] and p[b] are the row (y) offset numerators of its two texel
+// probes (a == b for a one-probe feature).  The trainer then lays each node's
+// candidates out along a Hilbert curve of (p[a], p[b]) before cutting them
+// into 64-lane tiles.  Images are row-major and a 128-byte cache line is a
+// strip of one row, so a tile whose probes share a narrow band of row
+// offsets touches few lines per gather, and the samples that follow (sorted
+// by pixel within an image) probe the same rows again while they are still
+// in the L1.  The order only decides which lane evaluates which candidate;
+// every count, key and output column stays that of the candidate's own index.
 // ---------------------------------------------------------------------------
 template <class F, class = void>
-struct probe_param : std::integral_constant<int, -1> {};
+struct probe_rows {
+  static constexpr bool on = false;
+  static constexpr int a = 0, b = 0;
+};
 template <class F>
-struct probe_param<F, std::void_t<decltype(F::kProbeParam)>> : std::integral_constant<int, F::kProbeParam> {};
+struct probe_rows<F, std::void_t<decltype(F::kProbeRowA), decltype(F::kProbeRowB)>> {
+  static constexpr bool on = true;
+  static constexpr int a = F::kProbeRowA, b = F::kProbeRowB;
+};
 
 constexpr uint32_t ORDER_MAX = 4096;   // candidates per node the order kernel sorts in LDS
 constexpr uint32_t ORDER_THREADS = 256;
 
-// 10-bit grid coordinate of v in [lo, hi] spread to the even bits
+// 10-bit grid coordinate of v in [lo, hi]
 template <typename T>
 __device__ __forceinline__ uint32_t order_coord(T v, T lo, T hi)
 {
   const float span = (float)hi - (float)lo;
   float t = span > 0.0f ? ((float)v - (float)lo) / span : 0.0f;
   t = fminf(fmaxf(t, 0.0f), 1.0f);
-  uint32_t q = (uint32_t)(t * 1023.0f);
-  q = (q | (q << 8)) & 0x00ff00ffu;
-  q = (q | (q << 4)) & 0x0f0f0f0fu;
-  q = (q | (q << 2)) & 0x33333333u;
-  q = (q | (q << 1)) & 0x55555555u;
-  return q;
+  return (uint32_t)(t * 1023.0f);
+}
+
+// Position of cell (x, y) of the 1024 x 1024 grid along a Hilbert curve (the
+// quadrant-by-quadrant xy -> d conversion with a rotation / reflection per
+// level).  Consecutive positions are neighbouring cells, so any run of 64
+// candidates covers a compact patch of the (row offset, row offset) plane.
+PD_HD uint32_t hilbert_index(uint32_t x, uint32_t y)
+{
+  uint32_t d = 0;
+  for (uint32_t s = 512u; s > 0u; s >>= 1) {
+    const uint32_t rx = (x & s) ? 1u : 0u, ry = (y & s) ? 1u : 0u;
+    d += s * s * ((3u * rx) ^ ry);
+    if (ry == 0u) {
+      if (rx == 1u) {
+        x = 1023u - x;
+        y = 1023u - y;
+      }
+      const uint32_t t = x;
+      x = y;
+      y = t;
+    }
+  }
+  return d;
 }
 
 // One block per slot: perm[q][j] = the candidate (output column) at tile
-// position j of slot q, sorted by (Morton key of the first probe's offset,
-// column): a bitonic sort of nF <= ORDER_MAX 64-bit keys in LDS.
+// position j of slot q, sorted by (Hilbert position of the probes' row
+// offsets, column): a bitonic sort of nF <= ORDER_MAX 64-bit keys in LDS.
 template <class F>
 __global__ __launch_bounds__(ORDER_THREADS) void k_cand_order(uint32_t tree, const int32_t *__restrict__ slot_node,
                                                               uint32_t nF, Bounds<F> bnd,
@@ -234,8 +262,9 @@ __global__ __launch_bounds__(ORDER_THREADS) void k_cand_order(uint32_t tree, con
                                                               uint16_t *__restrict__ perm)
 {
   using T = typename F::FeatType;
-  constexpr int J = probe_param<F>::value;
-  static_assert(J >= 0 && J + 1 < F::FeatDim, "kProbeParam names two parameters");
+  constexpr int A = probe_rows<F>::a, B = probe_rows<F>::b;
+  static_assert(probe_rows<F>::on && A >= 0 && A < F::FeatDim && B >= 0 && B < F::FeatDim,
+                "kProbeRowA / kProbeRowB name parameters of the feature");
   __shared__ unsigned long long key[ORDER_MAX];
   const uint32_t q = blockIdx.x;
   const uint32_t node = (uint32_t)slot_node[q];
@@ -247,8 +276,8 @@ __global__ __launch_bounds__(ORDER_THREADS) void k_cand_order(uint32_t tree, con
       const uint32_t fs = cand_map ? cand_map[(size_t)q * nF + f] : f;
       T p[F::FeatDim];
       candidate_params<T, F::FeatDim>(tree, node, fs, bnd.lo, bnd.hi, p);
-      const uint32_t m = order_coord<T>(p[J], bnd.lo[J], bnd.hi[J]) |
-                         (order_coord<T>(p[J + 1], bnd.lo[J + 1], bnd.hi[J + 1]) << 1);
+      const uint32_t qa = order_coord<T>(p[A], bnd.lo[A], bnd.hi[A]);
+      const uint32_t m = A == B ? qa : hilbert_index(qa, order_coord<T>(p[B], bnd.lo[B], bnd.hi[B]));
       k = ((unsigned long long)m << 32) | f;
     }
     key[f] = k;
