@@ -333,6 +333,7 @@ struct TrainerBase {
   uint64_t hist_budget = 8ull << 30;
   uint32_t chunk = 0;
   bool probe_order = true;   // probe-coherent candidate tiles (functors with kProbeRowA/B)
+  bool wide_bins = true;     // one 32-bit histogram bin per rank where it fits (WIDE_LDS_MAX)
   pd_tuning tune{8u, 16u};   // waves per histogram block, target blocks per CU
   std::vector<pd_tuning_rule> tune_rules;   // per level, by samples per node (decreasing)
   pd_tuning level_tuning(uint64_t active, uint32_t trainable) const
@@ -578,12 +579,28 @@ struct Trainer final : TrainerBase {
   void launch_hist_w(uint32_t n_items, uint32_t item_base, const uint4 *act, const Bounds<F> &bnd, uint32_t tree,
                      uint32_t nF, uint32_t nT)
   {
-    const size_t lds = hist_lds_bytes<F, NTP>(nC);
+#if !PD_KERNEL_SET_LIGHT
+    // one 32-bit bin per rank where the tile still fits two blocks per CU
+    if constexpr (sizeof(typename F::FeatType) == 4 && NW >= 8 && NTP <= 32) {
+      if (wide_bins && hist_lds_bytes<F, NTP, true>(nC) <= WIDE_LDS_MAX) {
+        launch_hist_k<NTP, NW, true>(n_items, item_base, act, bnd, tree, nF, nT);
+        return;
+      }
+    }
+#endif
+    launch_hist_k<NTP, NW, false>(n_items, item_base, act, bnd, tree, nF, nT);
+  }
+
+  template <int NTP, int NW, bool WIDE>
+  void launch_hist_k(uint32_t n_items, uint32_t item_base, const uint4 *act, const Bounds<F> &bnd, uint32_t tree,
+                     uint32_t nF, uint32_t nT)
+  {
+    const size_t lds = hist_lds_bytes<F, NTP, WIDE>(nC);
     if (lds > lds_limit) throw ArgError("histogram tile needs " + std::to_string(lds) + " B of LDS");
     if (lds > 65536)
-      HIPCHK(hipFuncSetAttribute((const void *)k_hist<F, NTP, NW>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)lds));
-    hipLaunchKernelGGL((k_hist<F, NTP, NW>), dim3(n_items), dim3(NW * 64), lds, stream, d_pixels.as<I>(),
+      HIPCHK(hipFuncSetAttribute((const void *)k_hist<F, NTP, NW, WIDE>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((k_hist<F, NTP, NW, WIDE>), dim3(n_items), dim3(NW * 64), lds, stream, d_pixels.as<I>(),
                        d_imgs.as<ImgDesc>(), act, d_items.as<HistItem>(), item_base, d_slot_node.as<int32_t>(),
                        d_slot_mode.as<uint8_t>(), d_slot_hidx.as<int32_t>(), d_tot.as<uint32_t>(),
                        d_slot_h.as<float>(), bnd, tree, nF,

@@ -183,16 +183,37 @@ struct DepthF32Feature {
   }
 };
 
-// probe_div with an integer depth (IntDivPrep).  Only the load is
-// predicated: lanes whose probe leaves the image issue no memory access.
-template <typename ImgT, int NCH>
-PD_HD float probe_int(const ImageView<ImgT, NCH> &im, int x, int y, const IntDivPrep &dp, float px, float py, int ch,
-                      float bg)
+// offset_of(q) for a quotient q = int_div(p, dp) with a positive
+// denominator: q then has the sign of the numerator p (zero included), so the
+// half added before truncation, copysign(0.5 - 2^-25, p), depends on the
+// candidate's parameter only and leaves the per-pixel work (the compiler
+// hoists it out of the sample loop).
+PD_HD int offset_of_num(float q, float p)
 {
-  const int ox = x + offset_of(int_div(px, dp));
-  const int oy = y + offset_of(int_div(py, dp));
-  float v = bg;
-  if ((uint32_t)ox < (uint32_t)im.w && (uint32_t)oy < (uint32_t)im.h) v = (float)im.at(ox, oy, ch);
+#if defined(__HIP_DEVICE_COMPILE__)
+  q = __builtin_amdgcn_fmed3f(q, -1048576.0f, 1048576.0f);
+  return (int)(q + copysignf(0.49999997f, p));
+#else
+  (void)p;
+  return offset_of(q);
+#endif
+}
+
+// Texel of channel `ch` at x + round(px/d), y + round(py/d) with an integer
+// depth d > 0 (IntDivPrep) and an integer pixel type, or `bg` when the offset
+// leaves the image; the value is returned as an int so that a hook's
+// difference of two probes is one integer subtraction (exact, as the float
+// one is for values below 2^24).  Only the load is predicated: lanes whose
+// probe leaves the image issue no memory access.
+template <typename ImgT, int NCH>
+PD_HD int probe_int(const ImageView<ImgT, NCH> &im, int x, int y, const IntDivPrep &dp, float px, float py, int ch,
+                    int bg)
+{
+  static_assert(sizeof(ImgT) <= 2 && ImgT(-1) > ImgT(0), "probe_int reads unsigned 8- or 16-bit pixels");
+  const int ox = x + offset_of_num(int_div(px, dp), px);
+  const int oy = y + offset_of_num(int_div(py, dp), py);
+  int v = bg;
+  if ((uint32_t)ox < (uint32_t)im.w && (uint32_t)oy < (uint32_t)im.h) v = (int)im.at(ox, oy, ch);
   return v;
 }
 
@@ -213,9 +234,9 @@ struct DepthU16Feature {
   PD_HD static float eval_prep(const ImageView<uint16_t, 1> &im, int x, int y, Prep dp, const float *p)
   {
     if (!(dp.d > 0.0f)) return 0.0f;
-    const float a = probe_int(im, x, y, dp, p[0], p[1], 0, 10000.0f);
-    const float b = probe_int(im, x, y, dp, p[2], p[3], 0, 10000.0f);
-    return a - b;
+    const int a = probe_int(im, x, y, dp, p[0], p[1], 0, 10000);
+    const int b = probe_int(im, x, y, dp, p[2], p[3], 0, 10000);
+    return (float)(a - b);
   }
   PD_HD static float eval(const ImageView<uint16_t, 1> &im, int x, int y, const float *p)
   {
@@ -248,9 +269,9 @@ struct RGBDFeature {
   PD_HD static float eval_prep(const ImageView<uint16_t, 4> &im, int x, int y, Prep dp, const float *p)
   {
     if (!(dp.d > 0.0f)) return 0.0f;
-    const float a = probe_int(im, x, y, dp, p[0], p[1], chan(p[4]), 10000.0f);
-    const float b = probe_int(im, x, y, dp, p[2], p[3], chan(p[5]), 10000.0f);
-    return a - b;
+    const int a = probe_int(im, x, y, dp, p[0], p[1], chan(p[4]), 10000);
+    const int b = probe_int(im, x, y, dp, p[2], p[3], chan(p[5]), 10000);
+    return (float)(a - b);
   }
   PD_HD static float eval(const ImageView<uint16_t, 4> &im, int x, int y, const float *p)
   {

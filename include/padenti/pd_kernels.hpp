@@ -67,6 +67,18 @@ constexpr int HIST_THREADS = 512;   // 8 wave64s share one 64-candidate tile
 template <int NTP>
 constexpr int rank_words() { return (NTP + 2) / 2; }
 
+// Rank bins come in two layouts: 16-bit halves of a word (ranks r and r+1 in
+// word r/2, the default) or, "wide", one 32-bit word per rank.  The wide
+// layout takes twice the LDS but turns the bin update into the sorted
+// threshold slot's own offset (two VALU instructions instead of eight), so
+// the trainer picks it where it still fits two blocks per CU
+// (WIDE_LDS_MAX) and the responses are 4-byte values (the threshold slot
+// and the bin word then have the same stride).
+template <int NTP, bool WIDE>
+constexpr int bin_rows() { return WIDE ? NTP + 1 : rank_words<NTP>(); }
+
+constexpr size_t WIDE_LDS_MAX = 80u * 1024u;
+
 // power-of-two slot count of the per-lane sorted threshold list
 template <int NTP>
 constexpr int sorted_slots() { return NTP <= 1 ? 1 : (NTP <= 8 ? 8 : (NTP <= 16 ? 16 : (NTP <= 32 ? 32 : 64))); }
@@ -74,18 +86,18 @@ constexpr int sorted_slots() { return NTP <= 1 ? 1 : (NTP <= 8 ? 8 : (NTP <= 16 
 // LDS words of the region shared by the rank bins and, before the bins are
 // zeroed, the candidate staging (parameters + raw thresholds): the staging
 // is dead once the thresholds are sorted, so the two alias.
-template <class F, int NTP>
+template <class F, int NTP, bool WIDE = false>
 constexpr size_t hist_front_words(uint32_t nC)
 {
-  const size_t bins = (size_t)nC * rank_words<NTP>() * 64;
+  const size_t bins = (size_t)nC * bin_rows<NTP, WIDE>() * 64;
   const size_t staging = ((size_t)(F::FeatDim + NTP) * 64 * sizeof(typename F::FeatType) + 3) / 4;
   return bins > staging ? bins : staging;
 }
 
-template <class F, int NTP>
+template <class F, int NTP, bool WIDE = false>
 constexpr size_t hist_lds_bytes(uint32_t nC)
 {
-  return hist_front_words<F, NTP>(nC) * 4 +                                 // rank bins | staging
+  return hist_front_words<F, NTP, WIDE>(nC) * 4 +                           // rank bins | staging
          (size_t)sorted_slots<NTP>() * 64 * sizeof(typename F::FeatType) + // sorted thresholds
          (size_t)NTP * 64;                                                   // ranks L_t
 }
@@ -129,13 +141,12 @@ __device__ __forceinline__ typename F::FeatType hist_eval(const typename F::ImgT
 // Rank r = #{t : !(v <= thr_t)} of response v by a branch-free binary search
 // over the lane's sorted thresholds (the predicate is true exactly on a
 // prefix of the sorted list; a NaN response makes it true everywhere), then
-// one LDS add into bin [class][r].
-template <class F, int NTP>
+// one LDS add into bin [class][r] (ranks above nT count as nT).
+template <class F, int NTP, bool WIDE>
 __device__ __forceinline__ void hist_add(typename F::FeatType v, uint4 r, const typename F::FeatType *sth,
                                          uint32_t nT, bool fv, uint32_t lane, uint32_t *bins)
 {
   using T = typename F::FeatType;
-  constexpr int RW = rank_words<NTP>();
   constexpr int NS = sorted_slots<NTP>();
   // binary search carried as the byte offset of slot `pos` in the lane's
   // column (slot stride 64 * sizeof(T)); the step offsets fold into the LDS
@@ -151,11 +162,24 @@ __device__ __forceinline__ void hist_add(typename F::FeatType v, uint4 r, const 
   // the steps reach at most NS-1: one more test of slot `pos` (the first
   // slot whose predicate can be false) adds the last one when all NS are true
   if (NS == NTP) at += (v <= *(const T *)at) ? 0u : SB;
-  const uint32_t pos = (uint32_t)(at - col) / SB;
-  const uint32_t rk = pos < nT ? pos : nT;
-  // class row of the bins: uniform (the record is wave-uniform)
-  const uint32_t crow = __builtin_amdgcn_readfirstlane(rec_cls(r) * (uint32_t)(RW * 64));
-  if (fv) atomicAdd(&bins[crow + (rk >> 1) * 64u + lane], 1u << ((rk & 1u) * 16u));
+  if constexpr (WIDE) {
+    static_assert(sizeof(T) == 4, "wide bins need 4-byte responses");
+    // slot pos of the lane's threshold column and word [pos][lane] of a
+    // class's bins are the same byte offset from their region's base
+    const char *top = col + nT * SB;
+    at = at < top ? at : top;
+    const uint32_t crow = __builtin_amdgcn_readfirstlane(rec_cls(r) * (uint32_t)((NTP + 1) * 64 * 4));
+    const ptrdiff_t k = ((const char *)bins - (const char *)sth) + (ptrdiff_t)crow;   // uniform
+    uint32_t *dst = (uint32_t *)(const_cast<char *>(at) + k);
+    if (fv) atomicAdd(dst, 1u);
+  } else {
+    constexpr int RW = rank_words<NTP>();
+    const uint32_t pos = (uint32_t)(at - col) / SB;
+    const uint32_t rk = pos < nT ? pos : nT;
+    // class row of the bins: uniform (the record is wave-uniform)
+    const uint32_t crow = __builtin_amdgcn_readfirstlane(rec_cls(r) * (uint32_t)(RW * 64));
+    if (fv) atomicAdd(&bins[crow + (rk >> 1) * 64u + lane], 1u << ((rk & 1u) * 16u));
+  }
 }
 
 // Class fraction p / d of a split side (counts, p <= d): the plain IEEE
@@ -187,8 +211,10 @@ constexpr uint8_t SLOT_FUSED = 2;    // one chunk: evaluate the gains in-kernel,
 // Gain of pair (t, f) read from the in-LDS prefix counts of this lane's
 // candidate: left(c) = prefix_c[L_t].  Op order of learn_best_feature.cl:63-93
 // (h: the node entropy, N: the node total, both computed per thread).
+template <bool WIDE>
 __device__ __forceinline__ uint32_t prefix_at(const uint32_t *bins, int RW, uint32_t c, uint32_t l, uint32_t lane)
 {
+  if constexpr (WIDE) return bins[(c * RW + l) * 64u + lane];
   return (bins[(c * RW + (l >> 1)) * 64u + lane] >> ((l & 1u) * 16u)) & 0xffffu;
 }
 
@@ -301,7 +327,7 @@ __global__ __launch_bounds__(ORDER_THREADS) void k_cand_order(uint32_t tree, con
     perm[(size_t)q * nF + j] = (uint16_t)(key[j] & 0xffffu);
 }
 
-template <class F, int NTP, int NWAVES = HIST_THREADS / 64>
+template <class F, int NTP, int NWAVES = HIST_THREADS / 64, bool WIDE = false>
 __global__ __launch_bounds__(NWAVES * 64) void k_hist(
     const typename F::ImgType *__restrict__ pixels, const ImgDesc *__restrict__ imgs,
     const uint4 *__restrict__ recs, const HistItem *__restrict__ items, uint32_t item_base,
@@ -314,7 +340,7 @@ __global__ __launch_bounds__(NWAVES * 64) void k_hist(
 {
   using T = typename F::FeatType;
   constexpr int FD = F::FeatDim;
-  constexpr int RW = rank_words<NTP>();
+  constexpr int RW = bin_rows<NTP, WIDE>();   // bin words per (class, lane)
   constexpr uint32_t NW = NWAVES;
   constexpr uint32_t NTH = NWAVES * 64;
   static_assert(NWAVES >= 2, "candidate set-up uses two waves");
@@ -323,7 +349,7 @@ __global__ __launch_bounds__(NWAVES * 64) void k_hist(
   uint32_t *bins = lds;           // [nC][RW][64], zeroed once the staging is dead
   T *sp = (T *)lds;               // staging: parameters [FD][64] ...
   T *st = sp + FD * 64;           // ... and raw thresholds [NTP][64]
-  T *sth = (T *)(lds + hist_front_words<F, NTP>(nC));
+  T *sth = (T *)(lds + hist_front_words<F, NTP, WIDE>(nC));
   uint8_t *sl = (uint8_t *)(sth + NS * 64);
 
   const HistItem it = items[item_base + blockIdx.x];
@@ -396,12 +422,12 @@ __global__ __launch_bounds__(NWAVES * 64) void k_hist(
   for (; i + NW < end; i += 2 * NW) {
     const uint4 ra = recs[i];
     const uint4 rb = recs[i + NW];
-    hist_add<F, NTP>(hist_eval<F>(pixels, imgs, ra, p, ac), ra, sth, nT, fv, lane, bins);
-    hist_add<F, NTP>(hist_eval<F>(pixels, imgs, rb, p, ac), rb, sth, nT, fv, lane, bins);
+    hist_add<F, NTP, WIDE>(hist_eval<F>(pixels, imgs, ra, p, ac), ra, sth, nT, fv, lane, bins);
+    hist_add<F, NTP, WIDE>(hist_eval<F>(pixels, imgs, rb, p, ac), rb, sth, nT, fv, lane, bins);
   }
   if (i < end) {
     const uint4 ra = recs[i];
-    hist_add<F, NTP>(hist_eval<F>(pixels, imgs, ra, p, ac), ra, sth, nT, fv, lane, bins);
+    hist_add<F, NTP, WIDE>(hist_eval<F>(pixels, imgs, ra, p, ac), ra, sth, nT, fv, lane, bins);
   }
   __syncthreads();
 
@@ -416,17 +442,22 @@ __global__ __launch_bounds__(NWAVES * 64) void k_hist(
 #pragma unroll
     for (int j = 0; j < RW; j++) {
       uint32_t w = b[j * 64];
-      uint32_t lo = run + (w & 0xffffu);
-      uint32_t hi = lo + (w >> 16);
-      run = hi;
-      b[j * 64] = lo | (hi << 16);
+      if constexpr (WIDE) {
+        run += w;
+        b[j * 64] = run;
+      } else {
+        uint32_t lo = run + (w & 0xffffu);
+        uint32_t hi = lo + (w >> 16);
+        run = hi;
+        b[j * 64] = lo | (hi << 16);
+      }
     }
     if (!fv || hidx < 0) continue;
     uint32_t *out = H + ((size_t)hidx * nC + c) * per_class + f;
 #pragma unroll
     for (int t = 0; t < NTP; t++) {
       if ((uint32_t)t >= nT) break;
-      const uint32_t val = prefix_at(bins, RW, c, sl[t * 64 + lane], lane);
+      const uint32_t val = prefix_at<WIDE>(bins, RW, c, sl[t * 64 + lane], lane);
       if (mode & SLOT_ATOMIC) {
         if (val) atomicAdd(out + (size_t)t * nF, val);
       } else {
@@ -455,13 +486,13 @@ __global__ __launch_bounds__(NWAVES * 64) void k_hist(
       for (uint32_t t = wave; t < nT; t += NW) {
         const uint32_t l = sl[t * 64 + lane];
         int nL = 0;
-        for (uint32_t c = 0; c < nC; c++) nL += (int)prefix_at(bins, RW, c, l, lane);
+        for (uint32_t c = 0; c < nC; c++) nL += (int)prefix_at<WIDE>(bins, RW, c, l, lane);
         const int nR = N - nL;
         const float fL = (float)nL, fR = (float)nR;
         const float yL = 1.0f / fL, yR = 1.0f / fR;   // used by the fast form only
         float hL = 0.0f, hR = 0.0f;
         for (uint32_t c = 0; c < nC; c++) {
-          const uint32_t lc = prefix_at(bins, RW, c, l, lane);
+          const uint32_t lc = prefix_at<WIDE>(bins, RW, c, l, lane);
           float tmp = nL ? count_ratio<decltype(fast)::value>(lc, fL, yL) : 0.0f;
           hL -= plogp(tmp);
           tmp = nR ? count_ratio<decltype(fast)::value>(tt[c] - lc, fR, yR) : 0.0f;
@@ -491,7 +522,7 @@ __global__ __launch_bounds__(NWAVES * 64) void k_hist(
     const uint32_t k = key_index(m);
     const uint32_t wl = (uint32_t)__builtin_ctzll(wmask);
     const uint32_t l = sl[(k / nF) * 64 + wl];
-    cand_left[ci * nC + tid] = prefix_at(bins, RW, tid, l, wl);
+    cand_left[ci * nC + tid] = prefix_at<WIDE>(bins, RW, tid, l, wl);
   }
 }
 

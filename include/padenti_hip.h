@@ -140,12 +140,19 @@ PD_API int pd_trainer_set_hist_budget(pd_trainer *tr, uint64_t bytes);
 /* Samples per histogram work item (0 = automatic). */
 PD_API int pd_trainer_set_chunk(pd_trainer *tr, uint32_t samples_per_chunk);
 /* Probe-coherent candidate tiles (default on): for feature hooks that name
- * their first probe's offset parameters, each node's candidates are handed
- * to the 64-lane histogram tiles in Morton order of that offset, so one
- * texel gather touches fewer cache lines.  Results are identical either
- * way.  No counterpart in the reference (hist_update.cl assigns candidate
- * f to work item f). */
+ * the row offset parameters of their probes, each node's candidates are
+ * handed to the 64-lane histogram tiles along a Hilbert curve of those
+ * offsets, so the gathers of a tile touch few cache lines and the samples
+ * that follow reuse them.  Results are identical either way.  No
+ * counterpart in the reference (hist_update.cl assigns candidate f to work
+ * item f). */
 PD_API int pd_trainer_set_probe_order(pd_trainer *tr, int enable);
+/* Histogram bin layout (default on = one 32-bit bin per threshold rank
+ * wherever that tile still fits two blocks per CU and the hook's responses
+ * are 4-byte values; off = two 16-bit bins per word everywhere).  Results
+ * are identical either way.  No counterpart in the reference (its
+ * histograms are per-sample global-memory increments, hist_update.cl). */
+PD_API int pd_trainer_set_wide_bins(pd_trainer *tr, int enable);
 
 /* Train one tree: TreeTrainer::train(tree, ts, params, 1, end_depth)
  * (tree_trainer.hpp:82-85).  Tree arrays are the reference SoA layout of a

@@ -210,6 +210,11 @@ class HIPTreeTrainer:
         """Probe-coherent candidate tiles on/off (results are identical)."""
         check(lib().pd_trainer_set_probe_order(self._h, 1 if enable else 0))
 
+    def set_wide_bins(self, enable):
+        """32-bit histogram bins per threshold rank where they fit (default) or
+        16-bit bins everywhere (results are identical)."""
+        check(lib().pd_trainer_set_wide_bins(self._h, 1 if enable else 0))
+
     def set_record_sink(self, buf, per_node):
         """Plain trainings emit per_node records per trainable node into the
         RecordBuffer `buf` (appending after its current count); None removes it."""

@@ -142,6 +142,7 @@ def lib():
         "pd_trainer_set_hist_budget": (I32, [VP, U64]),
         "pd_trainer_set_chunk": (I32, [VP, U32]),
         "pd_trainer_set_probe_order": (I32, [VP, I32]),
+        "pd_trainer_set_wide_bins": (I32, [VP, I32]),
         "pd_trainer_set_tuning": (I32, [VP, C.POINTER(Tuning)]),
         "pd_trainer_get_tuning": (I32, [VP, C.POINTER(Tuning)]),
         "pd_trainer_set_tuning_rules": (I32, [VP, U32, VP]),

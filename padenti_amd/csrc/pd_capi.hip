@@ -73,6 +73,14 @@ int pd_trainer_set_probe_order(pd_trainer *tr, int enable)
   });
 }
 
+int pd_trainer_set_wide_bins(pd_trainer *tr, int enable)
+{
+  return guarded([&] {
+    require(tr != nullptr, "null trainer");
+    tr->impl->wide_bins = enable != 0;
+  });
+}
+
 int pd_trainer_set_tuning(pd_trainer *tr, const pd_tuning *tuning)
 {
   return guarded([&] {

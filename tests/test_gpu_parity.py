@@ -285,6 +285,46 @@ def test_probe_order_tree_identical(gpu):
     assert np.array_equal(a.posteriors, b.posteriors)
 
 
+@pytest.mark.parametrize("kind,nF,nT,nC", [
+    (pa.PD_FEAT_DEPTH_F32, 150, 20, 5),   # NaN and +inf pixels: NaN responses rank above every threshold
+    (pa.PD_FEAT_DEPTH_U16, 200, 32, 8),   # widest tile that keeps 32-bit bins (8 x 33 ranks)
+    (pa.PD_FEAT_RGBD_U16X4, 100, 10, 3),
+    (pa.PD_FEAT_DEPTH_U16, 90, 20, 21),   # too many classes for 32-bit bins: 16-bit either way
+])
+def test_bin_layouts_identical(gpu, kind, nF, nT, nC):
+    """32-bit and 16-bit rank bins give the oracle's histograms, in fused
+    (single-chunk) and chunked nodes."""
+    rng = np.random.default_rng(500 + nF + nC)
+    ts = _tiny_ts(kind, rng, nC=nC)
+    if kind == pa.PD_FEAT_DEPTH_F32:
+        for im in ts.images:
+            px = im.data.reshape(im.height, im.width)
+            px[rng.integers(0, im.height, 12), rng.integers(0, im.width, 12)] = np.nan
+            px[rng.integers(0, im.height, 6), rng.integers(0, im.width, 6)] = np.inf
+    fd = L.FEATURES[kind][3]
+    if kind == pa.PD_FEAT_DEPTH_F32:
+        lo, hi, tb = [-3.0] * fd, [3.0] * fd, (-2.0, 2.0)
+    else:
+        lo, hi, tb = [-100000.0] * fd, [100000.0] * fd, (-800.0, 800.0)
+        if kind == pa.PD_FEAT_RGBD_U16X4:
+            lo[4:], hi[4:] = [0.0, 0.0], [4.0, 4.0]
+    p = pa.TreeTrainerParameters(nF, nT, lo, hi, tb[0], tb[1], 0)
+    S = sum(im.samples.size for im in ts.images)
+    nodes = np.array([0, 1, 2], dtype=np.int32)
+    sample_node = rng.choice([0, 1, 2, -1], S).astype(np.int32)
+    want = orc.hist_nodes(orc.Tset(kind, ts), p, 3, nodes, sample_node)
+    for wide in (True, False):
+        for chunk in (0, 50):
+            tr = pa.HIPTreeTrainer(kind, nC)
+            tr.set_training_set(ts)
+            tr.set_wide_bins(wide)
+            if chunk:
+                tr.set_chunk(chunk)
+            H = tr.hist_nodes(p, 3, nodes, sample_node)
+            tr.close()
+            assert np.array_equal(H, want), (wide, chunk)
+
+
 def test_split_search_device(gpu):
     rng = np.random.default_rng(7)
     for nC, nT, nF in [(4, 10, 100), (8, 20, 2000), (16, 32, 70), (21, 5, 9), (3, 2, 3)]:
