@@ -243,7 +243,7 @@ struct probe_rows<F, std::void_t<decltype(F::kProbeRowA), decltype(F::kProbeRowB
 };
 
 constexpr uint32_t ORDER_MAX = 4096;   // candidates per node the order kernel sorts in LDS
-constexpr uint32_t ORDER_THREADS = 256;
+constexpr uint32_t ORDER_THREADS = 1024;
 
 // 10-bit grid coordinate of v in [lo, hi]
 template <typename T>
