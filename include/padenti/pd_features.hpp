@@ -187,12 +187,18 @@ struct DepthF32Feature {
 // denominator: q then has the sign of the numerator p (zero included), so the
 // half added before truncation, copysign(0.5 - 2^-25, p), depends on the
 // candidate's parameter only and leaves the per-pixel work (the compiler
-// hoists it out of the sample loop).
+// hoists it out of the sample loop).  No clamp: v_cvt_i32_f32 saturates
+// (|q| >= 2^31 gives +-(2^31 - 1) or -2^31), and the caller adds the offset
+// in unsigned arithmetic, so every |q| >= 2^16 still lands outside any image
+// (width, height < 2^16), the decision the host's clamp to +-2^20 makes.  q
+// is never NaN (finite parameters, d >= 1).
 PD_HD int offset_of_num(float q, float p)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
-  q = __builtin_amdgcn_fmed3f(q, -1048576.0f, 1048576.0f);
-  return (int)(q + copysignf(0.49999997f, p));
+  const float t = q + copysignf(0.49999997f, p);
+  int r;
+  __asm__("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(t));
+  return r;
 #else
   (void)p;
   return offset_of(q);
@@ -210,10 +216,10 @@ PD_HD int probe_int(const ImageView<ImgT, NCH> &im, int x, int y, const IntDivPr
                     int bg)
 {
   static_assert(sizeof(ImgT) <= 2 && ImgT(-1) > ImgT(0), "probe_int reads unsigned 8- or 16-bit pixels");
-  const int ox = x + offset_of_num(int_div(px, dp), px);
-  const int oy = y + offset_of_num(int_div(py, dp), py);
+  const uint32_t ox = (uint32_t)x + (uint32_t)offset_of_num(int_div(px, dp), px);
+  const uint32_t oy = (uint32_t)y + (uint32_t)offset_of_num(int_div(py, dp), py);
   int v = bg;
-  if ((uint32_t)ox < (uint32_t)im.w && (uint32_t)oy < (uint32_t)im.h) v = (int)im.at(ox, oy, ch);
+  if (ox < (uint32_t)im.w && oy < (uint32_t)im.h) v = (int)im.at((int)ox, (int)oy, ch);
   return v;
 }
 

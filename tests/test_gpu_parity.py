@@ -158,6 +158,10 @@ def test_feature_eval_device(gpu, kind):
     prm = rng.uniform(-scale, scale, (n, fd))
     if kind == pa.PD_FEAT_RGBD_U16X4:
         prm[:, 4:] = rng.uniform(-0.5, 4.5, (n, 2))
+    if kind in (pa.PD_FEAT_DEPTH_U16, pa.PD_FEAT_RGBD_U16X4):
+        # offsets far outside any image, up to quotients beyond the int range
+        big = 10.0 ** rng.uniform(5, 38, (200, 4)) * rng.choice([-1.0, 1.0], (200, 4))
+        prm[50:250, :4] = big
     prm = prm.astype(L.FEATURES[kind][2])
     prm[:50] = np.round(prm[:50])       # exact .5 quotients occur with integral offsets
     out = np.zeros(n, dtype=L.FEATURES[kind][2])
