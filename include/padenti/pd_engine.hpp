@@ -708,9 +708,10 @@ struct Trainer final : TrainerBase {
     if (ev_begin) HIPCHK(hipEventRecord(ev_begin->e, stream));
     d_perm_cur = nullptr;
     if constexpr (probe_rows<F>::on) {
-      if (probe_order && nF > 64 && nF <= ORDER_MAX) {
+      if (probe_order && nF > 64 && nF <= 65536u) {
         d_perm.ensure((size_t)ns * nF * sizeof(uint16_t));
-        hipLaunchKernelGGL((k_cand_order<F>), dim3(ns), dim3(ORDER_THREADS), 0, stream, tree,
+        const uint32_t segs = (nF + ORDER_MAX - 1) / ORDER_MAX;
+        hipLaunchKernelGGL((k_cand_order<F>), dim3(ns, segs), dim3(ORDER_THREADS), 0, stream, tree,
                            d_slot_node.as<int32_t>(), nF, bnd, d_map_cur, d_perm.as<uint16_t>());
         HIPCHK(hipGetLastError());
         d_perm_cur = d_perm.as<uint16_t>();

@@ -242,7 +242,7 @@ struct probe_rows<F, std::void_t<decltype(F::kProbeRowA), decltype(F::kProbeRowB
   static constexpr int a = F::kProbeRowA, b = F::kProbeRowB;
 };
 
-constexpr uint32_t ORDER_MAX = 4096;   // candidates per node the order kernel sorts in LDS
+constexpr uint32_t ORDER_MAX = 4096;   // candidates one order block sorts in LDS (a segment of a node)
 constexpr uint32_t ORDER_THREADS = 1024;
 
 // 10-bit grid coordinate of v in [lo, hi]
@@ -278,9 +278,12 @@ PD_HD uint32_t hilbert_index(uint32_t x, uint32_t y)
   return d;
 }
 
-// One block per slot: perm[q][j] = the candidate (output column) at tile
-// position j of slot q, sorted by (Hilbert position of the probes' row
-// offsets, column): a bitonic sort of nF <= ORDER_MAX 64-bit keys in LDS.
+// One block per (slot q, segment g of ORDER_MAX candidates): perm[q][j] for
+// j in the segment = the candidate (output column) at tile position j,
+// the segment's candidates sorted by (Hilbert position of the probes' row
+// offsets, column): a bitonic sort of <= ORDER_MAX 64-bit keys in LDS.
+// Segments are whole tiles (ORDER_MAX is a multiple of 64) and perm holds
+// 16-bit columns, so nF <= 65536.
 template <class F>
 __global__ __launch_bounds__(ORDER_THREADS) void k_cand_order(uint32_t tree, const int32_t *__restrict__ slot_node,
                                                               uint32_t nF, Bounds<F> bnd,
@@ -292,14 +295,17 @@ __global__ __launch_bounds__(ORDER_THREADS) void k_cand_order(uint32_t tree, con
   static_assert(probe_rows<F>::on && A >= 0 && A < F::FeatDim && B >= 0 && B < F::FeatDim,
                 "kProbeRowA / kProbeRowB name parameters of the feature");
   __shared__ unsigned long long key[ORDER_MAX];
+  static_assert(ORDER_MAX % 64u == 0u && ORDER_MAX <= 65536u, "segments of whole tiles, 16-bit keys");
   const uint32_t q = blockIdx.x;
+  const uint32_t base = blockIdx.y * ORDER_MAX;
+  const uint32_t n = nF - base < ORDER_MAX ? nF - base : ORDER_MAX;   // the grid has base < nF
   const uint32_t node = (uint32_t)slot_node[q];
   uint32_t n2 = 1;
-  while (n2 < nF) n2 <<= 1;
+  while (n2 < n) n2 <<= 1;
   for (uint32_t f = threadIdx.x; f < n2; f += ORDER_THREADS) {
     unsigned long long k = ~0ull;   // padding sorts last
-    if (f < nF) {
-      const uint32_t fs = cand_map ? cand_map[(size_t)q * nF + f] : f;
+    if (f < n) {
+      const uint32_t fs = cand_map ? cand_map[(size_t)q * nF + base + f] : base + f;
       T p[F::FeatDim];
       candidate_params<T, F::FeatDim>(tree, node, fs, bnd.lo, bnd.hi, p);
       const uint32_t qa = order_coord<T>(p[A], bnd.lo[A], bnd.hi[A]);
@@ -323,8 +329,8 @@ __global__ __launch_bounds__(ORDER_THREADS) void k_cand_order(uint32_t tree, con
       __syncthreads();
     }
   }
-  for (uint32_t j = threadIdx.x; j < nF; j += ORDER_THREADS)
-    perm[(size_t)q * nF + j] = (uint16_t)(key[j] & 0xffffu);
+  for (uint32_t j = threadIdx.x; j < n; j += ORDER_THREADS)
+    perm[(size_t)q * nF + base + j] = (uint16_t)(base + (uint32_t)(key[j] & 0xffffu));
 }
 
 template <class F, int NTP, int NWAVES = HIST_THREADS / 64, bool WIDE = false>

@@ -177,11 +177,15 @@ class HIPTreeTrainer:
     def _apply_tuning(self, params):
         if not self._tuning:
             return
-        from .autotune import key, set_tuning
+        from .autotune import DEFAULT_TUNING, key, set_tuning
         e = self._tuning.get(key(self.kind, params.n_thresholds, self.n_classes))
         if e:
             rules = [(r["min_samples_per_node"], r["waves"], r["blocks_per_cu"]) for r in e.get("rules", [])]
             set_tuning(self, e["waves"], e["blocks_per_cu"], rules)
+        else:
+            # a shape the table does not hold runs with the engine's defaults,
+            # not with whatever an earlier train() of another shape applied
+            set_tuning(self, *DEFAULT_TUNING)
 
     def close(self):
         if getattr(self, "_h", None):

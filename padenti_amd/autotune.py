@@ -64,6 +64,9 @@ def load_table(path=TABLE):
         return {e["key"]: e for e in json.load(f).get("entries", [])}
 
 
+DEFAULT_TUNING = (8, 16)   # the engine's own launch tuning (waves per block, work items per CU)
+
+
 def set_tuning(trainer, waves, blocks_per_cu, rules=()):
     """Base tuning + per-level rules [(min_samples_per_node, waves, blocks_per_cu)], decreasing."""
     L.check(L.lib().pd_trainer_set_tuning(trainer._h, ctypes.byref(L.Tuning(waves, blocks_per_cu))))
@@ -162,7 +165,7 @@ def tune_workload(cfg_name, n_images=None, reps=2, log=None):
     res = sweep(tr, p, shapes, reps, log)
     total = {k: sum(v.values()) for k, v in res.items()}
     best = min(total, key=total.get)
-    default = total[(8, 16)]
+    default = total[DEFAULT_TUNING]
     rules = rules_from_sweep(res, shapes, S)
     per_shape = sum(min(v[n] for v in res.values()) for n in shapes)
 
@@ -186,8 +189,8 @@ def tune_workload(cfg_name, n_images=None, reps=2, log=None):
     # per-level rules measured on real tree levels: the six best tunings of
     # the sweep and the default, one tree each
     cands = sorted(total, key=total.get)[:6]
-    if (8, 16) not in cands:
-        cands.append((8, 16))
+    if DEFAULT_TUNING not in cands:
+        cands.append(DEFAULT_TUNING)
     lv = {t: tree_levels(tr, ts, p, c, t) for t in cands}
     levels = [(lv[best][i][0], {t: lv[t][i][1] for t in cands}) for i in range(len(lv[best]))]
     tbase, trules = rules_from_levels(levels, best)
@@ -208,7 +211,7 @@ def tune_workload(cfg_name, n_images=None, reps=2, log=None):
     choice = next(name for t, name in cand_t if t <= 1.02 * fastest)
     shape_rules = rules
     if choice == "default":
-        best, rules = (8, 16), []
+        best, rules = DEFAULT_TUNING, []
     elif choice == "single":
         rules = []
     elif choice == "tree_rules":

@@ -74,6 +74,27 @@ def test_autotune_keys_and_table(tmp_path):
     assert autotune.load_table(str(path))["1:20:8"]["waves"] == 4
 
 
+def test_tuning_table_shapes_without_entry_get_defaults(monkeypatch):
+    """A train() whose shape has no table entry runs with the engine's
+    default tuning, not with the entry an earlier train() applied."""
+    from padenti_amd import autotune
+    calls = []
+    monkeypatch.setattr(autotune, "set_tuning", lambda tr, w, b, rules=(): calls.append((w, b, list(rules))))
+
+    class Fake:
+        pass
+    t = Fake()
+    t.kind, t.n_classes = pa.PD_FEAT_DEPTH_U16, 8
+    t._tuning = {autotune.key(pa.PD_FEAT_DEPTH_U16, 20, 8):
+                 {"waves": 16, "blocks_per_cu": 32, "rules": [{"min_samples_per_node": 1000, "waves": 8,
+                                                               "blocks_per_cu": 64}]}}
+    b = [-100.0] * 4, [100.0] * 4
+    pa.HIPTreeTrainer._apply_tuning(t, pa.TreeTrainerParameters(2000, 20, b[0], b[1], -1.0, 1.0, 10))
+    pa.HIPTreeTrainer._apply_tuning(t, pa.TreeTrainerParameters(20000, 1, b[0], b[1], -1.0, 1.0, 10))
+    assert calls == [(16, 32, [(1000, 8, 64)]), (8, 16, [])]
+    assert autotune.DEFAULT_TUNING == (8, 16)
+
+
 def test_autotune_rules_from_sweep():
     from padenti_amd import autotune
     shapes = (1, 16, 256)

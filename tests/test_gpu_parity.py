@@ -238,6 +238,7 @@ def test_hist_nodes_device(gpu, kind, nF, nT, chunk):
     (pa.PD_FEAT_DEPTH_U16, 200, 20),      # 4 tiles, the last one partial
     (pa.PD_FEAT_DEMO_S16, 129, 6),        # the single-probe demo hook, 1-lane last tile
     (pa.PD_FEAT_RGBD_U16X4, 300, 9),
+    (pa.PD_FEAT_DEPTH_U16, 4200, 3),      # two order segments (4096 + 104 candidates)
 ])
 def test_probe_order_histograms_identical(gpu, kind, nF, nT):
     """Probe-coherent candidate tiles change which lane evaluates which
