@@ -330,6 +330,28 @@ def test_bin_layouts_identical(gpu, kind, nF, nT, nC):
             assert np.array_equal(H, want), (wide, chunk)
 
 
+def test_bin_layouts_tree_identical(gpu):
+    """A C2-shaped tree (fused in-kernel split search reading the prefix
+    counts, chunked nodes through H) is the same with 32-bit and 16-bit
+    rank bins."""
+    ts = synth.training_set("c2", n_images=6, samples=300)
+    p = synth.params_for("c2", n_features=300)
+    trees = []
+    for wide in (True, False):
+        tree = pa.Tree(2, 9, pa.PD_FEAT_DEPTH_U16, 8)
+        tr = pa.HIPTreeTrainer(pa.PD_FEAT_DEPTH_U16, 8, tuned=False)
+        tr.set_training_set(ts)
+        tr.set_wide_bins(wide)
+        tr.set_chunk(256)
+        tr.train(tree, ts, p, 1, 9)
+        tr.close()
+        trees.append(tree)
+    a, b = trees
+    assert np.array_equal(a.left_children, b.left_children)
+    assert np.array_equal(a.features, b.features) and a.thresholds.tobytes() == b.thresholds.tobytes()
+    assert a.posteriors.tobytes() == b.posteriors.tobytes()
+
+
 def test_split_search_device(gpu):
     rng = np.random.default_rng(7)
     for nC, nT, nF in [(4, 10, 100), (8, 20, 2000), (16, 32, 70), (21, 5, 9), (3, 2, 3)]:
