@@ -17,7 +17,7 @@
 //  * the waves stride over the chunk's samples; a sample is wave-uniform, so
 //    its record and image descriptor come through scalar loads;
 //  * for each (sample, candidate) the lane counts r = #{t : !(v <= thr_t)}
-//    and does ONE LDS add into rank bin [c][r] (16-bit halves of a u32);
+//    and does ONE LDS add into rank bin [c][r] (a 32-bit or 16-bit count);
 //  * at the end, count(c, t) = sum_{r <= L_t} bin[c][r] with
 //    L_t = #{t' : thr_t' < thr_t}, which equals #{samples : v <= thr_t}
 //    exactly (proof in the kernel comment), and is stored (single-chunk
@@ -63,19 +63,20 @@ __device__ __forceinline__ int rec_y(uint4 r) { return (int)(r.y >> 16); }
 
 constexpr int HIST_THREADS = 512;   // 8 wave64s share one 64-candidate tile
 
-// words per (class, lane) of the 16-bit rank bins for ranks 0..NTP
-template <int NTP>
-constexpr int rank_words() { return (NTP + 2) / 2; }
-
-// Rank bins come in two layouts: 16-bit halves of a word (ranks r and r+1 in
-// word r/2, the default) or, "wide", one 32-bit word per rank.  The wide
-// layout takes twice the LDS but turns the bin update into the sorted
-// threshold slot's own offset (two VALU instructions instead of eight), so
-// the trainer picks it where it still fits two blocks per CU
-// (WIDE_LDS_MAX) and the responses are 4-byte values (the threshold slot
-// and the bin word then have the same stride).
+// Rank bins [class][rank 0..NTP][lane] come in two layouts: 16-bit counts
+// (two neighbouring lanes share a word) or, "wide", one 32-bit word per
+// (rank, lane).  Either way the bin of rank r is found from the binary
+// search's final threshold slot address (slot r of the lane's column) by a
+// shift and a uniform offset, not by rank arithmetic.  The wide layout takes
+// twice the LDS and needs 4-byte responses (slot and bin word then have the
+// same stride); the trainer picks it where it still fits two blocks per CU
+// (WIDE_LDS_MAX).
 template <int NTP, bool WIDE>
-constexpr int bin_rows() { return WIDE ? NTP + 1 : rank_words<NTP>(); }
+constexpr int bin_rows() { return NTP + 1; }
+
+// 32-bit words per rank row of one class (64 lanes)
+template <bool WIDE>
+constexpr int bin_row_words() { return WIDE ? 64 : 32; }
 
 constexpr size_t WIDE_LDS_MAX = 80u * 1024u;
 
@@ -89,7 +90,7 @@ constexpr int sorted_slots() { return NTP <= 1 ? 1 : (NTP <= 8 ? 8 : (NTP <= 16 
 template <class F, int NTP, bool WIDE = false>
 constexpr size_t hist_front_words(uint32_t nC)
 {
-  const size_t bins = (size_t)nC * bin_rows<NTP, WIDE>() * 64;
+  const size_t bins = (size_t)nC * bin_rows<NTP, WIDE>() * bin_row_words<WIDE>();
   const size_t staging = ((size_t)(F::FeatDim + NTP) * 64 * sizeof(typename F::FeatType) + 3) / 4;
   return bins > staging ? bins : staging;
 }
@@ -162,23 +163,26 @@ __device__ __forceinline__ void hist_add(typename F::FeatType v, uint4 r, const 
   // the steps reach at most NS-1: one more test of slot `pos` (the first
   // slot whose predicate can be false) adds the last one when all NS are true
   if (NS == NTP) at += (v <= *(const T *)at) ? 0u : SB;
+  const char *top = col + nT * SB;   // ranks above nT count as nT
+  at = at < top ? at : top;
+  // class row of the bins, in bytes: uniform (the record is wave-uniform)
+  const uint32_t crow =
+      __builtin_amdgcn_readfirstlane(rec_cls(r) * (uint32_t)((NTP + 1) * bin_row_words<WIDE>() * 4));
   if constexpr (WIDE) {
     static_assert(sizeof(T) == 4, "wide bins need 4-byte responses");
     // slot pos of the lane's threshold column and word [pos][lane] of a
     // class's bins are the same byte offset from their region's base
-    const char *top = col + nT * SB;
-    at = at < top ? at : top;
-    const uint32_t crow = __builtin_amdgcn_readfirstlane(rec_cls(r) * (uint32_t)((NTP + 1) * 64 * 4));
     const ptrdiff_t k = ((const char *)bins - (const char *)sth) + (ptrdiff_t)crow;   // uniform
     uint32_t *dst = (uint32_t *)(const_cast<char *>(at) + k);
     if (fv) atomicAdd(dst, 1u);
   } else {
-    constexpr int RW = rank_words<NTP>();
-    const uint32_t pos = (uint32_t)(at - col) / SB;
-    const uint32_t rk = pos < nT ? pos : nT;
-    // class row of the bins: uniform (the record is wave-uniform)
-    const uint32_t crow = __builtin_amdgcn_readfirstlane(rec_cls(r) * (uint32_t)(RW * 64));
-    if (fv) atomicAdd(&bins[crow + (rk >> 1) * 64u + lane], 1u << ((rk & 1u) * 16u));
+    // the 16-bit count [pos][lane] sits at byte 2 * (64 pos + lane): the
+    // slot's byte offset (64 pos + lane) * sizeof(T) rescaled; its word is
+    // shared with the neighbouring lane (low half: even lanes)
+    const uint32_t off = (uint32_t)(at - (const char *)sth);
+    const uint32_t hoff = sizeof(T) >= 2 ? off / (uint32_t)(sizeof(T) / 2) : off * 2u;
+    uint32_t *dst = (uint32_t *)((char *)bins + crow + (hoff & ~3u));
+    if (fv) atomicAdd(dst, 1u << ((lane & 1u) * 16u));
   }
 }
 
@@ -215,7 +219,7 @@ template <bool WIDE>
 __device__ __forceinline__ uint32_t prefix_at(const uint32_t *bins, int RW, uint32_t c, uint32_t l, uint32_t lane)
 {
   if constexpr (WIDE) return bins[(c * RW + l) * 64u + lane];
-  return (bins[(c * RW + (l >> 1)) * 64u + lane] >> ((l & 1u) * 16u)) & 0xffffu;
+  return ((const uint16_t *)bins)[(c * RW + l) * 64u + lane];
 }
 
 // ---------------------------------------------------------------------------
@@ -346,13 +350,13 @@ __global__ __launch_bounds__(NWAVES * 64) void k_hist(
 {
   using T = typename F::FeatType;
   constexpr int FD = F::FeatDim;
-  constexpr int RW = bin_rows<NTP, WIDE>();   // bin words per (class, lane)
+  constexpr int RW = bin_rows<NTP, WIDE>();   // rank rows per class
   constexpr uint32_t NW = NWAVES;
   constexpr uint32_t NTH = NWAVES * 64;
   static_assert(NWAVES >= 2, "candidate set-up uses two waves");
   constexpr int NS = sorted_slots<NTP>();
   extern __shared__ __align__(16) uint32_t lds[];
-  uint32_t *bins = lds;           // [nC][RW][64], zeroed once the staging is dead
+  uint32_t *bins = lds;           // [nC][RW][64] counts, zeroed once the staging is dead
   T *sp = (T *)lds;               // staging: parameters [FD][64] ...
   T *st = sp + FD * 64;           // ... and raw thresholds [NTP][64]
   T *sth = (T *)(lds + hist_front_words<F, NTP, WIDE>(nC));
@@ -413,7 +417,7 @@ __global__ __launch_bounds__(NWAVES * 64) void k_hist(
 #pragma unroll
   for (int j = 0; j < FD; j++) p[j] = sp[j * 64 + lane];
   __syncthreads();   // staging read by every wave: the bins may now take its place
-  for (uint32_t i = tid; i < nC * RW * 64u; i += NTH) bins[i] = 0u;
+  for (uint32_t i = tid; i < nC * (uint32_t)(RW * bin_row_words<WIDE>()); i += NTH) bins[i] = 0u;
   __syncthreads();
 
   // Sample loop, two samples per wave and step.  r(v) = #{t : !(v <= thr_t)}
@@ -443,19 +447,20 @@ __global__ __launch_bounds__(NWAVES * 64) void k_hist(
   const int32_t hidx = slot_hidx[it.slot];
   const size_t per_class = (size_t)nT * nF;
   for (uint32_t c = wave; c < nC; c += NW) {
-    uint32_t *b = bins + (size_t)c * RW * 64u + lane;
     uint32_t run = 0;
+    if constexpr (WIDE) {
+      uint32_t *b = bins + (size_t)c * RW * 64u + lane;
 #pragma unroll
-    for (int j = 0; j < RW; j++) {
-      uint32_t w = b[j * 64];
-      if constexpr (WIDE) {
-        run += w;
+      for (int j = 0; j < RW; j++) {
+        run += b[j * 64];
         b[j * 64] = run;
-      } else {
-        uint32_t lo = run + (w & 0xffffu);
-        uint32_t hi = lo + (w >> 16);
-        run = hi;
-        b[j * 64] = lo | (hi << 16);
+      }
+    } else {
+      uint16_t *b = (uint16_t *)bins + (size_t)c * RW * 64u + lane;
+#pragma unroll
+      for (int j = 0; j < RW; j++) {
+        run += b[j * 64];
+        b[j * 64] = (uint16_t)run;
       }
     }
     if (!fv || hidx < 0) continue;
