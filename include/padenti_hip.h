@@ -149,7 +149,7 @@ PD_API int pd_trainer_set_chunk(pd_trainer *tr, uint32_t samples_per_chunk);
 PD_API int pd_trainer_set_probe_order(pd_trainer *tr, int enable);
 /* Histogram bin layout (default on = one 32-bit bin per threshold rank
  * wherever that tile still fits two blocks per CU and the hook's responses
- * are 4-byte values; off = two 16-bit bins per word everywhere).  Results
+ * are 4-byte values; off = 16-bit bins, two lanes per word, everywhere).  Results
  * are identical either way.  No counterpart in the reference (its
  * histograms are per-sample global-memory increments, hist_update.cl). */
 PD_API int pd_trainer_set_wide_bins(pd_trainer *tr, int enable);
