@@ -136,6 +136,10 @@ public:
     const pd_tuning t = {waves, blocksPerCU};
     pd_cxx::check(pd_trainer_set_tuning(m_tr, &t));
   }
+  // Histogram layout knobs; trees are identical whatever they are set to.
+  void setChunk(uint32_t samplesPerChunk) { pd_cxx::check(pd_trainer_set_chunk(m_tr, samplesPerChunk)); }
+  void setProbeOrder(bool enable) { pd_cxx::check(pd_trainer_set_probe_order(m_tr, enable ? 1 : 0)); }
+  void setWideBins(bool enable) { pd_cxx::check(pd_trainer_set_wide_bins(m_tr, enable ? 1 : 0)); }
   // One trainer call in three: per-node launches, level-batched (default),
   // or level-batched with the learned split proposal (setProposal first).
   void setMode(int mode) { m_mode = mode; }

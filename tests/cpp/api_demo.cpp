@@ -102,6 +102,17 @@ int main(int argc, char **argv)
         fprintf(stderr, "tuned launch trained a different tree\n");
         return 1;
       }
+      // 16-bit bins, no candidate order, small chunks: the same tree again
+      TreeT layout(0, 6);
+      TrainerT t3(PD_FEAT_DEPTH_F32);
+      t3.setWideBins(false);
+      t3.setProbeOrder(false);
+      t3.setChunk(128);
+      t3.train(layout, ts, params, 1, 6);
+      if (!same_tree(tree, layout)) {
+        fprintf(stderr, "histogram layout knobs trained a different tree\n");
+        return 1;
+      }
     }
     // learned split proposal: records from a plain training, the network
     // trained on them, and a proposal that keeps every candidate (= plain)
